@@ -1,0 +1,253 @@
+/*
+ * swegl_hip.h -- C ABI of the MI355X renderer (libswegl_hip.so).
+ *
+ * This is the drop-in boundary for swegl's CPU render call
+ *   template<typename...T> void swegl::render(scene_t&, T&... viewports)
+ *   (swegl/render/renderer.hpp:27-34, src/render/renderer.cpp:77-235).
+ * The reference call is a header-only C++ template with no ABI; the entry
+ * points below are what an FFI for that path binds: plain structs, pointers
+ * and sizes, integer status codes, no exceptions and no C++ types.
+ *
+ * Conventions
+ *  - Every function returns 0 (SWEGL_OK) or a negative errno-style code.
+ *  - Work is enqueued on the context's HIP stream; results are valid after
+ *    swegl_ctx_synchronize() (or a blocking read-back call).
+ *  - Matrices are row-major float[16] with the reference's meaning:
+ *    m[r*4+c] == matrix44_t[r][c] (swegl/projection/matrix44.hpp:13-44).
+ *  - Colours are swegl::pixel_colors stored as a 32-bit word: byte 0 = b,
+ *    1 = g, 2 = r, 3 = a (swegl/render/colors.hpp:9-33).
+ *  - One context serves one host thread at a time (the reference render is
+ *    not re-entrant either: stateful shaders, mutated scene).
+ */
+#ifndef SWEGL_HIP_H
+#define SWEGL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SWEGL_OK          0
+#define SWEGL_EIO        (-5)   /* HIP runtime error (message via swegl_last_error) */
+#define SWEGL_EAGAIN     (-11)  /* admission: not enough budget / slots right now */
+#define SWEGL_ENOMEM     (-12)  /* device allocation failed / over HBM budget */
+#define SWEGL_ENODEV     (-19)  /* no usable gfx950 device */
+#define SWEGL_EINVAL     (-22)  /* malformed input (indices out of range, bad rect, ...) */
+#define SWEGL_ERANGE     (-34)  /* a fixed capacity was exceeded (bins, queue) */
+#define SWEGL_ENOTSUP    (-95)  /* shader/post-shader/transparency mode not on the GPU path */
+
+/* ---- opaque handles ---------------------------------------------------- */
+typedef struct swegl_ctx    swegl_ctx;
+typedef struct swegl_scene  swegl_scene;
+typedef struct swegl_target swegl_target;
+typedef struct swegl_server swegl_server;
+
+/* ---- scene description (swegl/data/model.hpp) --------------------------- */
+
+/* primitive_t::index_mode_t (model.hpp:33-42); only triangle modes render */
+#define SWEGL_TRIANGLES       4
+#define SWEGL_TRIANGLE_STRIP  5
+#define SWEGL_TRIANGLE_FAN    6
+
+/* primitive_t (model.hpp:31-48) with mesh_vertex_t inputs (model.hpp:20-29)
+ * given as separate arrays: v (xyz), normal (xyz), tex_coords (xy). */
+typedef struct swegl_primitive_desc {
+    uint32_t        n_vertices;
+    const float    *positions;   /* n_vertices*3 : mesh_vertex_t::v          */
+    const float    *normals;     /* n_vertices*3 : mesh_vertex_t::normal     */
+    const float    *tex_coords;  /* n_vertices*2 : mesh_vertex_t::tex_coords */
+    uint32_t        n_indices;
+    const uint32_t *indices;     /* primitive_t::indices                     */
+    int32_t         mode;        /* SWEGL_TRIANGLES / _STRIP / _FAN          */
+    int32_t         material_id; /* index into materials, -1 = none          */
+} swegl_primitive_desc;
+
+/* node_t (model.hpp:50-71).  Its primitives are
+ * primitives[first_primitive .. first_primitive+n_primitives) and its
+ * children are child_indices[first_child .. first_child+n_children). */
+typedef struct swegl_node_desc {
+    float    scale[3];           /* node_t::scale        */
+    float    rotation[16];       /* node_t::rotation     */
+    float    translation[3];     /* node_t::translation  */
+    uint32_t first_primitive, n_primitives;
+    uint32_t first_child, n_children;
+} swegl_node_desc;
+
+/* material_t (model.hpp:80-87) */
+typedef struct swegl_material_desc {
+    uint32_t color;              /* pixel_colors word */
+    float    metallic, roughness;
+    int32_t  texture_idx;        /* -1 = none */
+    int32_t  double_sided;
+} swegl_material_desc;
+
+/* texture_t mip 0 (swegl/data/texture.hpp:12-38): row-major width*height
+ * pixel_colors words. */
+typedef struct swegl_texture_desc {
+    const uint32_t *bitmap;
+    uint32_t        width, height;
+} swegl_texture_desc;
+
+/* point_source_light (model.hpp:74-78) */
+typedef struct swegl_point_light {
+    float position[3];
+    float intensity;
+} swegl_point_light;
+
+/* scene_t (model.hpp:128-178).  Draw order is nodes[] order, as
+ * _render iterates scene.nodes (renderer.cpp:83). */
+typedef struct swegl_scene_desc {
+    uint32_t                    n_nodes;
+    const swegl_node_desc      *nodes;
+    uint32_t                    n_root_nodes;
+    const int32_t              *root_nodes;
+    uint32_t                    n_child_indices;
+    const int32_t              *child_indices;
+    uint32_t                    n_primitives;
+    const swegl_primitive_desc *primitives;
+    uint32_t                    n_materials;
+    const swegl_material_desc  *materials;
+    swegl_material_desc         default_material;
+    uint32_t                    n_textures;
+    const swegl_texture_desc   *textures;
+    uint32_t                    n_point_lights;
+    const swegl_point_light    *point_lights;
+    float                       ambient_light_intensity;
+    float                       sun_direction[3];   /* normal_t */
+    float                       sun_intensity;
+} swegl_scene_desc;
+
+/* ---- viewport (swegl/render/viewport.hpp:27-62) ------------------------- */
+
+/* pixel shader plugins (swegl/render/pixel_shaders.hpp).  The GPU path
+ * implements pixel_shader_light_and_texture<pixel_shader_lights_phong,
+ * pixel_shader_texture_bilinear> (pixel_shaders.hpp:121-179). */
+#define SWEGL_SHADER_PHONG_BILINEAR  1
+
+/* post shaders (swegl/render/post_shaders.hpp) */
+#define SWEGL_POST_NULL        0   /* post_shader_t, no transparency: no-op (:33-47)   */
+#define SWEGL_POST_DEPTH_BOX   1   /* post_shader_depth_box: box taps (:51-131)          */
+#define SWEGL_POST_DEPTH_TAP9  2   /* same blur radius, 3x3 taps at {-r,0,r}^2           */
+
+typedef struct swegl_viewport_desc {
+    int32_t       x, y, w, h;        /* viewport_t::m_x, m_y, m_w, m_h           */
+    swegl_target *color;             /* the SDL_Surface (viewport_t::m_screen)    */
+    swegl_target *depth;             /* viewport_t::m_zbuffer, w*h floats         */
+    swegl_target *triangle_id;       /* optional (may be NULL): per-pixel winning
+                                        draw sequence, w*h uint32, 0xFFFFFFFF = none */
+    float         view[16];          /* camera_t::m_viewmatrix                    */
+    float         projection[16];    /* camera_t::m_projectionmatrix              */
+    float         camera_position[3];/* camera_t::m_center (camera_t::position()) */
+    int32_t       pixel_shader;      /* SWEGL_SHADER_*                            */
+    int32_t       post_shader;       /* SWEGL_POST_*                              */
+    float         focal_distance;    /* post_shader_depth_box::focal_distance     */
+    float         focal_depth;       /* post_shader_depth_box::focal_depth        */
+    int32_t       dof_max_radius;    /* 5 in the reference (post_shaders.hpp:69)  */
+    int32_t       transparency_layer_count; /* must be 0 on the GPU path          */
+} swegl_viewport_desc;
+
+/* Per-render counters, filled when the render's work completes. */
+typedef struct swegl_frame_stats {
+    uint64_t fragments;          /* (sub-triangle, pixel) samples with z > 0.001 */
+    uint64_t nan_fragments;      /* samples whose z is NaN (order-dependent in
+                                    the reference; must be 0 for exact parity) */
+    uint64_t drawn_subtriangles; /* fill_triangle_2 calls that reach the DDA     */
+    uint64_t bin_entries;        /* (triangle, tile) pairs                        */
+    uint64_t texel_oob;          /* bilinear taps outside the bitmap (UB in the
+                                    reference; clamped here)                      */
+} swegl_frame_stats;
+
+/* ---- context ----------------------------------------------------------- */
+int  swegl_ctx_create(int device, swegl_ctx **out);
+int  swegl_ctx_destroy(swegl_ctx *ctx);
+int  swegl_ctx_synchronize(swegl_ctx *ctx);
+const char *swegl_last_error(void);
+/* Device facts: number of CUs, gcn arch name (e.g. "gfx950:sramecc+:xnack-") */
+int  swegl_device_info(swegl_ctx *ctx, int *n_cus, char *arch, int arch_cap,
+                       uint64_t *hbm_bytes);
+
+/* Raster tuning: tile edge in pixels (8, 16, 32 or 64) and the bin
+ * capacity hint; the autotuner writes these. */
+int  swegl_ctx_set_tile(swegl_ctx *ctx, int tile_w, int tile_h);
+int  swegl_ctx_get_tile(swegl_ctx *ctx, int *tile_w, int *tile_h);
+
+/* Per-kernel HIP-event timing on the context stream (0 = off). */
+int  swegl_ctx_set_profiling(swegl_ctx *ctx, int on);
+/* Average milliseconds per launch of each kernel since profiling was
+ * switched on: names[i] (NUL-separated into name_buf), ms[i], counts[i]. */
+int  swegl_ctx_kernel_times(swegl_ctx *ctx, char *name_buf, int name_cap,
+                            double *ms, uint64_t *counts, int cap, int *n);
+
+/* ---- device-resident surfaces ----------------------------------------- */
+/* A colour surface (width*height pixel_colors words, pitch = width*4) or a
+ * float/uint32 plane of width*height elements (depth, triangle id). */
+#define SWEGL_TARGET_COLOR  0
+#define SWEGL_TARGET_FLOAT  1
+#define SWEGL_TARGET_UINT   2
+int  swegl_target_create(swegl_ctx *ctx, int kind, int32_t width, int32_t height,
+                         swegl_target **out);
+int  swegl_target_destroy(swegl_ctx *ctx, swegl_target *t);
+/* Blocking copies between a target and host memory with the given row pitch
+ * in bytes (SDL_Surface::pitch). */
+int  swegl_target_read(swegl_ctx *ctx, swegl_target *t, void *dst, int32_t dst_pitch);
+int  swegl_target_write(swegl_ctx *ctx, swegl_target *t, const void *src, int32_t src_pitch);
+/* Enqueue a fill of every element with a 32-bit pattern. */
+int  swegl_target_fill(swegl_ctx *ctx, swegl_target *t, uint32_t word);
+
+/* ---- scenes ------------------------------------------------------------ */
+/* Copy meshes, textures, materials, lights and poses to the device.  The
+ * scene stays resident until swegl_scene_destroy. */
+int  swegl_scene_upload(swegl_ctx *ctx, const swegl_scene_desc *desc, swegl_scene **out);
+/* Re-pose a resident scene: node scale/rotation/translation, lights,
+ * ambient and sun from desc (same node/primitive topology; meshes and
+ * textures are not re-sent). */
+int  swegl_scene_update_pose(swegl_ctx *ctx, swegl_scene *scene, const swegl_scene_desc *desc);
+int  swegl_scene_destroy(swegl_ctx *ctx, swegl_scene *scene);
+/* Device bytes held by a scene (meshes, textures, per-frame scratch). */
+int  swegl_scene_bytes(swegl_ctx *ctx, swegl_scene *scene, uint64_t *bytes);
+
+/* ---- render: swegl::render(scene, vp...) ------------------------------- */
+/* Replaces renderer.hpp:27-34: world transform once, then each viewport in
+ * order (renderer.cpp:77-235), including clear (viewport.cpp:88-113) and the
+ * post shader.  stats may be NULL. */
+int  swegl_render(swegl_ctx *ctx, swegl_scene *scene,
+                  const swegl_viewport_desc *viewports, uint32_t n_viewports,
+                  swegl_frame_stats *stats);
+
+/* Plain path as one replayed hipGraph: the first call for a given
+ * (scene, viewport set) captures, later calls replay. */
+int  swegl_render_graph(swegl_ctx *ctx, swegl_scene *scene,
+                        const swegl_viewport_desc *viewports, uint32_t n_viewports,
+                        swegl_frame_stats *stats);
+
+/* ---- batched: many independent jobs in one launch sequence ------------- */
+typedef struct swegl_job {
+    swegl_scene               *scene;
+    const swegl_viewport_desc *viewports;
+    uint32_t                   n_viewports;
+} swegl_job;
+int  swegl_render_batch(swegl_ctx *ctx, const swegl_job *jobs, uint32_t n_jobs,
+                        swegl_frame_stats *stats);
+
+/* ---- host math helpers (same float semantics as the reference) --------- */
+/* matrix44_t::rotate_x/y/z (src/projection/matrix44.cpp:12-67), translate
+ * (:87-92), swegl::scale (src/projection/points.cpp:14-30). */
+void swegl_matrix_rotate_x(float m[16], float a);
+void swegl_matrix_rotate_y(float m[16], float a);
+void swegl_matrix_rotate_z(float m[16], float a);
+void swegl_matrix_translate(float m[16], float x, float y, float z);
+/* camera_t(aspect) (src/projection/camera.cpp:8-35): view, projection,
+ * center. */
+void swegl_camera_init(float aspect_ratio, float view[16], float projection[16],
+                       float center[3]);
+/* camera_t::translate (camera.cpp:50-58) and camera_t::rotate_x/y/z
+ * (camera.cpp:37-48). axis: 0=x, 1=y, 2=z. */
+void swegl_camera_translate(float view[16], float center[3], float x, float y, float z);
+void swegl_camera_rotate(float view[16], int axis, float a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWEGL_HIP_H */

@@ -1,0 +1,118 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from swegl_amd import _abi as A
+from swegl_amd.host import SceneDescHolder
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class OracleViewport(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("w", C.c_int32), ("h", C.c_int32),
+                ("surface", A.U32P), ("surface_w", C.c_int32), ("surface_h", C.c_int32),
+                ("pitch", C.c_int32), ("zbuffer", A.FP), ("triangle_id", A.U32P),
+                ("view", C.c_float * 16), ("projection", C.c_float * 16),
+                ("camera_position", C.c_float * 3), ("post_shader", C.c_int32),
+                ("focal_distance", C.c_float), ("focal_depth", C.c_float),
+                ("dof_max_radius", C.c_int32)]
+
+
+class OracleStats(C.Structure):
+    _fields_ = [("fragments", C.c_uint64), ("nan_fragments", C.c_uint64),
+                ("drawn_subtriangles", C.c_uint64), ("texel_oob", C.c_uint64),
+                ("clipped_triangles", C.c_uint64)]
+
+
+def build():
+    subprocess.run(["make", "-B", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.oracle_render.restype = C.c_int
+        L.oracle_render.argtypes = [C.POINTER(A.SceneDesc), C.POINTER(OracleViewport), C.c_uint32,
+                                    C.POINTER(OracleStats)]
+        L.oracle_transform.argtypes = [A.FP, A.FP, A.FP]
+        L.oracle_vertex.argtypes = [A.FP] * 9
+        L.oracle_interp.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, A.FP]
+        L.oracle_raster_triangle.restype = C.c_int
+        L.oracle_raster_triangle.argtypes = [A.FP, C.c_int, C.c_int, A.U32P, A.FP]
+        L.oracle_bilinear.restype = C.c_uint32
+        L.oracle_bilinear.argtypes = [A.U32P, C.c_int, C.c_int, C.c_float, C.c_float]
+        L.oracle_phong.restype = C.c_int32
+        L.oracle_phong.argtypes = [A.FP, A.FP, A.FP, C.c_float, A.FP, C.c_float,
+                                   C.POINTER(A.PointLight), C.c_int]
+        L.oracle_combine.restype = C.c_uint32
+        L.oracle_combine.argtypes = [C.c_uint32, C.c_int32]
+        L.oracle_dof.argtypes = [A.U32P, A.FP, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
+                                 C.c_int, A.U32P]
+        L.oracle_cvtt.restype = C.c_int32
+        L.oracle_cvtt.argtypes = [C.c_float]
+        _lib = L
+    return _lib
+
+
+class Frame:
+    """Oracle output of one viewport: surface, zbuffer (h, w), triangle ids."""
+
+    def __init__(self, pixels, zbuffer, tid, stats):
+        self.pixels, self.zbuffer, self.tid, self.stats = pixels, zbuffer, tid, stats
+
+
+def render(scene, viewports, surfaces=None):
+    """swegl::render(scene, vp...) on the CPU oracle.  Each viewport renders
+    into a copy of its surface's current pixels (or `surfaces[i]`)."""
+    holder = SceneDescHolder(scene)
+    n = len(viewports)
+    ovs = (OracleViewport * n)()
+    keep = []
+    shared = {}
+    for i, vp in enumerate(viewports):
+        s = vp.m_screen
+        if surfaces is not None:
+            pix = surfaces[i]
+        else:
+            key = id(s)
+            if key not in shared:
+                shared[key] = s.pixels.copy()
+            pix = shared[key]
+        z = np.zeros((vp.m_h, vp.m_w), np.float32)
+        tid = np.zeros((vp.m_h, vp.m_w), np.uint32)
+        keep += [pix, z, tid]
+        o = ovs[i]
+        o.x, o.y, o.w, o.h = vp.m_x, vp.m_y, vp.m_w, vp.m_h
+        o.surface = A.u32ptr(pix)
+        o.surface_w, o.surface_h, o.pitch = s.w, s.h, s.pitch
+        o.zbuffer = A.fptr(z)
+        o.triangle_id = A.u32ptr(tid)
+        o.view[:] = [float(v) for v in vp.m_camera.m_viewmatrix.reshape(16)]
+        o.projection[:] = [float(v) for v in vp.m_camera.m_projectionmatrix.reshape(16)]
+        o.camera_position[:] = [float(v) for v in vp.m_camera.m_center]
+        ps = vp.m_post_shader
+        o.post_shader = ps.kind if ps is not None else -1
+        if ps is not None and ps.kind != A.POST_NULL:
+            o.focal_distance, o.focal_depth = ps.focal_distance, ps.focal_depth
+            o.dof_max_radius = ps.max_radius
+    st = (OracleStats * n)()
+    rc = lib().oracle_render(C.byref(holder.desc), ovs, n, st)
+    if rc != 0:
+        raise RuntimeError(f"oracle_render failed: {A.ERRNAMES.get(rc, rc)}")
+    frames = []
+    for i in range(n):
+        d = {k: getattr(st[i], k) for k, _ in OracleStats._fields_}
+        frames.append(Frame(keep[3 * i], keep[3 * i + 1], keep[3 * i + 2], d))
+    return frames

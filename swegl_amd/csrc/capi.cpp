@@ -1,0 +1,922 @@
+// capi.cpp -- C ABI of libswegl_hip.so (include/swegl_hip.h): contexts,
+// resident scenes and surfaces, and the launch sequence of one render.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/swegl_hip.h"
+#include "hostmath.h"
+#include "jobs.h"
+#include "launch.h"
+
+using namespace sw;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int rc, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return rc;
+}
+
+#define HIPCHK(x)                                                                   \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess)                                                       \
+            return fail(SWEGL_EIO, "%s: %s", #x, hipGetErrorString(e_));            \
+    } while (0)
+
+// grow-only device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int ensure(size_t bytes)
+    {
+        if (bytes <= n && p)
+            return SWEGL_OK;
+        release();
+        size_t want = std::max<size_t>(bytes, 256);
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            return fail(SWEGL_ENOMEM, "hipMalloc(%zu) failed", want);
+        }
+        n = want;
+        return SWEGL_OK;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+struct swegl_target {
+    int kind;
+    int32_t w, h;
+    DevBuf buf;
+};
+
+struct ViewScratch {
+    DevBuf ndc, yes, rect, big, dof;
+};
+
+struct swegl_scene {
+    DevBuf px, py, pz, nx, ny, nz, tu, tv, tri, prims, nodes, texels, texs, lights, world, nworld;
+    uint32_t n_vertices = 0, n_tris = 0, n_prims = 0, n_lights = 0;
+    std::vector<PrimDev> h_prims;
+    std::vector<swegl_node_desc> h_nodes;
+    std::vector<int32_t> roots, children;
+    std::vector<uint32_t> prim_node;   // node of each primitive
+    float ambient = 0, sun_I = 0;
+    f3 sun = {0, 0, 0};
+    std::vector<std::unique_ptr<ViewScratch>> views;
+    uint64_t resident = 0;
+};
+
+struct ProfRec {
+    const char *name;
+    hipEvent_t a, b;
+};
+
+struct StageSlot {
+    void *host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool used = false;
+};
+
+struct swegl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int tw = 32, th = 32;
+    DevBuf jobs, tile_count, tile_offset, tile_cursor, bins, ctr;
+    unsigned long long *h_stats = nullptr;   // pinned
+    StageSlot stage[4];
+    int stage_next = 0;
+    bool prof = false;
+    std::vector<ProfRec> pending;
+    std::map<std::string, std::pair<double, uint64_t>> prof_acc;
+    std::vector<hipEvent_t> event_pool;
+};
+
+// --------------------------------------------------------------- helpers
+static hipEvent_t take_event(swegl_ctx *c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+static void drain_profile(swegl_ctx *c)
+{
+    for (ProfRec &r : c->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto &acc = c->prof_acc[r.name];
+            acc.first += ms;
+            acc.second += 1;
+        }
+        c->event_pool.push_back(r.a);
+        c->event_pool.push_back(r.b);
+    }
+    c->pending.clear();
+}
+
+// Runs `launch` on the context stream, bracketed by events when profiling.
+template <class F>
+static hipError_t timed(swegl_ctx *c, const char *name, bool capturing, F launch)
+{
+    if (!c->prof || capturing)
+        return launch();
+    ProfRec r{name, take_event(c), take_event(c)};
+    (void)hipEventRecord(r.a, c->stream);
+    hipError_t e = launch();
+    (void)hipEventRecord(r.b, c->stream);
+    c->pending.push_back(r);
+    return e;
+}
+
+// A pinned staging slot for host->device uploads issued on the stream; a
+// slot is reused only after the copy that read it has completed.
+static int stage_acquire(swegl_ctx *c, size_t bytes, StageSlot **out)
+{
+    StageSlot &s = c->stage[c->stage_next];
+    c->stage_next = (c->stage_next + 1) % 4;
+    if (s.used)
+        HIPCHK(hipEventSynchronize(s.done));
+    if (s.cap < bytes) {
+        if (s.host)
+            (void)hipHostFree(s.host);
+        s.host = nullptr;
+        size_t want = std::max<size_t>(bytes, 4096);
+        HIPCHK(hipHostMalloc(&s.host, want, hipHostMallocDefault));
+        s.cap = want;
+    }
+    *out = &s;
+    return SWEGL_OK;
+}
+
+static int upload(swegl_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes)
+        return SWEGL_OK;
+    StageSlot *s;
+    int rc = stage_acquire(c, bytes, &s);
+    if (rc)
+        return rc;
+    memcpy(s->host, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, s->host, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(s->done, c->stream));
+    s->used = true;
+    return SWEGL_OK;
+}
+
+// ------------------------------------------------------------------ ctx
+extern "C" const char *swegl_last_error(void) { return g_err.c_str(); }
+
+extern "C" int swegl_ctx_create(int device, swegl_ctx **out)
+{
+    if (!out)
+        return fail(SWEGL_EINVAL, "null out");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
+        return fail(SWEGL_ENODEV, "no HIP device %d (count %d)", device, n);
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SWEGL_ENODEV, "device %d is %s, this build targets gfx950", device,
+                    prop.gcnArchName);
+    std::unique_ptr<swegl_ctx> c(new swegl_ctx);
+    c->device = device;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc((void **)&c->h_stats, 32 * sizeof(unsigned long long), hipHostMallocDefault));
+    for (StageSlot &s : c->stage)
+        HIPCHK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    *out = c.release();
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_ctx_destroy(swegl_ctx *c)
+{
+    if (!c)
+        return SWEGL_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    drain_profile(c);
+    for (hipEvent_t e : c->event_pool)
+        (void)hipEventDestroy(e);
+    for (StageSlot &s : c->stage) {
+        if (s.host)
+            (void)hipHostFree(s.host);
+        if (s.done)
+            (void)hipEventDestroy(s.done);
+    }
+    if (c->h_stats)
+        (void)hipHostFree(c->h_stats);
+    c->jobs.release();
+    c->tile_count.release();
+    c->tile_offset.release();
+    c->tile_cursor.release();
+    c->bins.release();
+    c->ctr.release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_ctx_synchronize(swegl_ctx *c)
+{
+    if (!c)
+        return fail(SWEGL_EINVAL, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_profile(c);
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_device_info(swegl_ctx *c, int *n_cus, char *arch, int arch_cap, uint64_t *hbm)
+{
+    if (!c)
+        return fail(SWEGL_EINVAL, "null ctx");
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->device));
+    if (n_cus)
+        *n_cus = prop.multiProcessorCount;
+    if (arch && arch_cap > 0)
+        snprintf(arch, arch_cap, "%s", prop.gcnArchName);
+    if (hbm)
+        *hbm = prop.totalGlobalMem;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_ctx_set_tile(swegl_ctx *c, int tw, int th)
+{
+    if (!c)
+        return fail(SWEGL_EINVAL, "null ctx");
+    static const int ok[][2] = {{8, 8}, {16, 16}, {32, 16}, {32, 32}, {64, 32}, {64, 64}};
+    for (auto &p : ok)
+        if (p[0] == tw && p[1] == th) {
+            c->tw = tw;
+            c->th = th;
+            return SWEGL_OK;
+        }
+    return fail(SWEGL_EINVAL, "unsupported tile %dx%d", tw, th);
+}
+
+extern "C" int swegl_ctx_get_tile(swegl_ctx *c, int *tw, int *th)
+{
+    if (!c)
+        return fail(SWEGL_EINVAL, "null ctx");
+    if (tw)
+        *tw = c->tw;
+    if (th)
+        *th = c->th;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_ctx_set_profiling(swegl_ctx *c, int on)
+{
+    if (!c)
+        return fail(SWEGL_EINVAL, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_profile(c);
+    c->prof_acc.clear();
+    c->prof = on != 0;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_ctx_kernel_times(swegl_ctx *c, char *names, int name_cap, double *ms,
+                                      uint64_t *counts, int cap, int *n)
+{
+    if (!c)
+        return fail(SWEGL_EINVAL, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_profile(c);
+    int i = 0, pos = 0;
+    for (auto &kv : c->prof_acc) {
+        if (i >= cap)
+            break;
+        int len = (int)kv.first.size() + 1;
+        if (pos + len > name_cap)
+            break;
+        memcpy(names + pos, kv.first.c_str(), len);
+        pos += len;
+        ms[i] = kv.second.second ? kv.second.first / (double)kv.second.second : 0.0;
+        counts[i] = kv.second.second;
+        i++;
+    }
+    if (n)
+        *n = i;
+    return SWEGL_OK;
+}
+
+// ---------------------------------------------------------------- targets
+extern "C" int swegl_target_create(swegl_ctx *c, int kind, int32_t w, int32_t h, swegl_target **out)
+{
+    if (!c || !out)
+        return fail(SWEGL_EINVAL, "null argument");
+    if (w <= 0 || h <= 0 || w > 32768 || h > 32768 || kind < 0 || kind > 2)
+        return fail(SWEGL_EINVAL, "bad target %dx%d kind %d", w, h, kind);
+    std::unique_ptr<swegl_target> t(new swegl_target);
+    t->kind = kind;
+    t->w = w;
+    t->h = h;
+    int rc = t->buf.ensure((size_t)w * h * 4);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemsetAsync(t->buf.p, 0, (size_t)w * h * 4, c->stream));
+    *out = t.release();
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_target_destroy(swegl_ctx *c, swegl_target *t)
+{
+    if (!t)
+        return SWEGL_OK;
+    if (c)
+        (void)hipStreamSynchronize(c->stream);
+    delete t;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_target_read(swegl_ctx *c, swegl_target *t, void *dst, int32_t pitch)
+{
+    if (!c || !t || !dst || pitch < t->w * 4)
+        return fail(SWEGL_EINVAL, "bad read arguments");
+    HIPCHK(hipMemcpy2DAsync(dst, pitch, t->buf.p, (size_t)t->w * 4, (size_t)t->w * 4, t->h,
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_profile(c);
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_target_write(swegl_ctx *c, swegl_target *t, const void *src, int32_t pitch)
+{
+    if (!c || !t || !src || pitch < t->w * 4)
+        return fail(SWEGL_EINVAL, "bad write arguments");
+    HIPCHK(hipMemcpy2DAsync(t->buf.p, (size_t)t->w * 4, src, pitch, (size_t)t->w * 4, t->h,
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SWEGL_OK;
+}
+
+namespace sw {
+hipError_t launch_fill32(uint32_t *p, uint64_t n, uint32_t word, hipStream_t s);
+}
+
+extern "C" int swegl_target_fill(swegl_ctx *c, swegl_target *t, uint32_t word)
+{
+    if (!c || !t)
+        return fail(SWEGL_EINVAL, "null argument");
+    HIPCHK(launch_fill32(t->buf.as<uint32_t>(), (uint64_t)t->w * t->h, word, c->stream));
+    return SWEGL_OK;
+}
+
+// ----------------------------------------------------------------- scenes
+// Node matrices: original_to_world = parent * get_local_world_matrix()
+// (vertex_shaders.hpp:16-28, model.hpp:56-61) and the normal matrix
+// scale(rotation, scale) (vertex_shaders.hpp:63).
+static int node_matrices(swegl_scene *s, std::vector<NodeDev> &out)
+{
+    const size_t nn = s->h_nodes.size();
+    out.assign(nn, NodeDev{});
+    std::vector<float> o2w(nn * 16);
+    std::vector<char> seen(nn, 0);
+    struct Item { int node; std::vector<float> parent; };
+    std::vector<Item> stack;
+    for (auto it = s->roots.rbegin(); it != s->roots.rend(); ++it)
+        stack.push_back({*it, std::vector<float>(hm::kIdentity, hm::kIdentity + 16)});
+    size_t guard = 0;
+    while (!stack.empty()) {
+        Item it = std::move(stack.back());
+        stack.pop_back();
+        if (++guard > 64 * (nn + 1))
+            return fail(SWEGL_EINVAL, "node hierarchy has a cycle");
+        const swegl_node_desc &d = s->h_nodes[it.node];
+        float local[16], rs[16], m[16];
+        hm::local_matrix(d.rotation, d.scale, d.translation, local, rs);
+        hm::matmul(it.parent.data(), local, m);
+        memcpy(&o2w[(size_t)it.node * 16], m, sizeof m);
+        seen[it.node] = 1;
+        NodeDev &nd = out[it.node];
+        memcpy(nd.M.m, m, 12 * sizeof(float));
+        memcpy(nd.RS.m, rs, 12 * sizeof(float));
+        for (int k = (int)d.n_children - 1; k >= 0; k--)
+            stack.push_back({s->children[d.first_child + k], std::vector<float>(m, m + 16)});
+    }
+    for (size_t i = 0; i < nn; i++)
+        if (!seen[i])
+            return fail(SWEGL_EINVAL, "node %zu is not reachable from a root node", i);
+    return SWEGL_OK;
+}
+
+static int scene_pose(swegl_ctx *c, swegl_scene *s, const swegl_scene_desc *d)
+{
+    s->h_nodes.assign(d->nodes, d->nodes + d->n_nodes);
+    s->roots.assign(d->root_nodes, d->root_nodes + d->n_root_nodes);
+    s->children.assign(d->child_indices, d->child_indices + d->n_child_indices);
+    for (int32_t r : s->roots)
+        if (r < 0 || (uint32_t)r >= d->n_nodes)
+            return fail(SWEGL_EINVAL, "root node %d out of range", r);
+    for (const swegl_node_desc &nd : s->h_nodes) {
+        if ((uint64_t)nd.first_child + nd.n_children > d->n_child_indices)
+            return fail(SWEGL_EINVAL, "child range out of range");
+        for (uint32_t k = 0; k < nd.n_children; k++) {
+            int32_t ci = d->child_indices[nd.first_child + k];
+            if (ci < 0 || (uint32_t)ci >= d->n_nodes)
+                return fail(SWEGL_EINVAL, "child index %d out of range", ci);
+        }
+    }
+    std::vector<NodeDev> nodes;
+    int rc = node_matrices(s, nodes);
+    if (rc)
+        return rc;
+    std::vector<Light4> lights(std::max<uint32_t>(1, d->n_point_lights));
+    for (uint32_t i = 0; i < d->n_point_lights; i++)
+        lights[i] = Light4{d->point_lights[i].position[0], d->point_lights[i].position[1],
+                           d->point_lights[i].position[2], d->point_lights[i].intensity};
+    s->n_lights = d->n_point_lights;
+    s->ambient = d->ambient_light_intensity;
+    s->sun = f3{d->sun_direction[0], d->sun_direction[1], d->sun_direction[2]};
+    s->sun_I = d->sun_intensity;
+    if ((rc = s->nodes.ensure(nodes.size() * sizeof(NodeDev))) ||
+        (rc = s->lights.ensure(lights.size() * sizeof(Light4))))
+        return rc;
+    if ((rc = upload(c, s->nodes.p, nodes.data(), nodes.size() * sizeof(NodeDev))) ||
+        (rc = upload(c, s->lights.p, lights.data(), lights.size() * sizeof(Light4))))
+        return rc;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_scene_upload(swegl_ctx *c, const swegl_scene_desc *d, swegl_scene **out)
+{
+    if (!c || !d || !out)
+        return fail(SWEGL_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    std::unique_ptr<swegl_scene> s(new swegl_scene);
+    // ---- validate and flatten primitives in draw order (nodes, then primitives)
+    std::vector<float> px, py, pz, nx, ny, nz, tu, tv;
+    std::vector<uint32_t> tri;
+    for (uint32_t ni = 0; ni < d->n_nodes; ni++) {
+        const swegl_node_desc &nd = d->nodes[ni];
+        if ((uint64_t)nd.first_primitive + nd.n_primitives > d->n_primitives)
+            return fail(SWEGL_EINVAL, "node %u primitive range out of range", ni);
+        for (uint32_t k = 0; k < nd.n_primitives; k++) {
+            const swegl_primitive_desc &p = d->primitives[nd.first_primitive + k];
+            if (p.mode != SWEGL_TRIANGLES && p.mode != SWEGL_TRIANGLE_STRIP && p.mode != SWEGL_TRIANGLE_FAN)
+                return fail(SWEGL_ENOTSUP, "primitive mode %d is not a triangle mode", p.mode);
+            if (p.material_id < 0 || (uint32_t)p.material_id >= d->n_materials)
+                return fail(SWEGL_EINVAL,
+                            "material_id %d: the bilinear shader indexes materials[material_id] "
+                            "(pixel_shaders.cpp:288-290)", p.material_id);
+            const swegl_material_desc &m = d->materials[p.material_id];
+            if (m.texture_idx < -1 || m.texture_idx >= (int)d->n_textures)
+                return fail(SWEGL_EINVAL, "texture_idx %d out of range", m.texture_idx);
+            if (p.n_vertices && (!p.positions || !p.normals || !p.tex_coords))
+                return fail(SWEGL_EINVAL, "null vertex arrays");
+            for (uint32_t i = 0; i < p.n_indices; i++)
+                if (p.indices[i] >= p.n_vertices)
+                    return fail(SWEGL_EINVAL, "index %u out of range (%u vertices)", p.indices[i],
+                                p.n_vertices);
+            PrimDev pd;
+            pd.first_vertex = (uint32_t)px.size();
+            pd.n_vertices = p.n_vertices;
+            pd.first_tri = (uint32_t)(tri.size() / 3);
+            pd.node = ni;
+            pd.tex = m.texture_idx;
+            pd.color = m.color;
+            pd.double_sided = m.double_sided ? 1 : 0;
+            const uint32_t base = pd.first_vertex;
+            for (uint32_t v = 0; v < p.n_vertices; v++) {
+                px.push_back(p.positions[3 * v]);
+                py.push_back(p.positions[3 * v + 1]);
+                pz.push_back(p.positions[3 * v + 2]);
+                nx.push_back(p.normals[3 * v]);
+                ny.push_back(p.normals[3 * v + 1]);
+                nz.push_back(p.normals[3 * v + 2]);
+                tu.push_back(p.tex_coords[2 * v]);
+                tv.push_back(p.tex_coords[2 * v + 1]);
+            }
+            const uint32_t *I = p.indices;
+            const size_t n = p.n_indices;
+            // draw order and winding of renderer.cpp:197-229
+            if (p.mode == SWEGL_TRIANGLE_STRIP)
+                for (size_t i = 2; i < n; i++) {
+                    tri.push_back(base + I[i - 2]);
+                    tri.push_back(base + I[i - 1 + (i & 1)]);
+                    tri.push_back(base + I[i - (i & 1)]);
+                }
+            if (p.mode == SWEGL_TRIANGLE_FAN)
+                for (size_t i = 2; i < n; i++) {
+                    tri.push_back(base + I[0]);
+                    tri.push_back(base + I[i - 1]);
+                    tri.push_back(base + I[i]);
+                }
+            if (p.mode == SWEGL_TRIANGLES)
+                for (size_t i = 2; i < n; i += 3) {
+                    tri.push_back(base + I[i - 2]);
+                    tri.push_back(base + I[i - 1]);
+                    tri.push_back(base + I[i]);
+                }
+            pd.n_tris = (uint32_t)(tri.size() / 3) - pd.first_tri;
+            s->h_prims.push_back(pd);
+        }
+    }
+    if (px.size() > 0x7FFFFFFFu || tri.size() / 3 > 0x7FFFFFFFu)
+        return fail(SWEGL_EINVAL, "scene too large");
+    s->n_vertices = (uint32_t)px.size();
+    s->n_tris = (uint32_t)(tri.size() / 3);
+    s->n_prims = (uint32_t)s->h_prims.size();
+    // ---- textures
+    std::vector<TexDev> texs(std::max<uint32_t>(1, d->n_textures));
+    uint64_t texels = 0;
+    for (uint32_t i = 0; i < d->n_textures; i++) {
+        const swegl_texture_desc &t = d->textures[i];
+        if (!t.bitmap || t.width == 0 || t.height == 0 || t.width > 65536 || t.height > 65536)
+            return fail(SWEGL_EINVAL, "texture %u is empty or too large", i);
+        texs[i] = TexDev{texels, (int32_t)t.width, (int32_t)t.height};
+        texels += (uint64_t)t.width * t.height;
+    }
+    int rc;
+    const size_t V = std::max<size_t>(1, px.size());
+    if ((rc = s->px.ensure(V * 4)) || (rc = s->py.ensure(V * 4)) || (rc = s->pz.ensure(V * 4)) ||
+        (rc = s->nx.ensure(V * 4)) || (rc = s->ny.ensure(V * 4)) || (rc = s->nz.ensure(V * 4)) ||
+        (rc = s->tu.ensure(V * 4)) || (rc = s->tv.ensure(V * 4)) ||
+        (rc = s->world.ensure(V * 16)) || (rc = s->nworld.ensure(V * 16)) ||
+        (rc = s->tri.ensure(std::max<size_t>(3, tri.size()) * 4)) ||
+        (rc = s->prims.ensure(std::max<size_t>(1, s->h_prims.size()) * sizeof(PrimDev))) ||
+        (rc = s->texs.ensure(texs.size() * sizeof(TexDev))) ||
+        (rc = s->texels.ensure(std::max<uint64_t>(1, texels) * 4)))
+        return rc;
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
+        if (!bytes)
+            return SWEGL_OK;
+        hipError_t e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess)
+            return fail(SWEGL_EIO, "upload: %s", hipGetErrorString(e));
+        return SWEGL_OK;
+    };
+    if ((rc = up(s->px, px.data(), px.size() * 4)) || (rc = up(s->py, py.data(), py.size() * 4)) ||
+        (rc = up(s->pz, pz.data(), pz.size() * 4)) || (rc = up(s->nx, nx.data(), nx.size() * 4)) ||
+        (rc = up(s->ny, ny.data(), ny.size() * 4)) || (rc = up(s->nz, nz.data(), nz.size() * 4)) ||
+        (rc = up(s->tu, tu.data(), tu.size() * 4)) || (rc = up(s->tv, tv.data(), tv.size() * 4)) ||
+        (rc = up(s->tri, tri.data(), tri.size() * 4)) ||
+        (rc = up(s->prims, s->h_prims.data(), s->h_prims.size() * sizeof(PrimDev))) ||
+        (rc = up(s->texs, texs.data(), texs.size() * sizeof(TexDev))))
+        return rc;
+    for (uint32_t i = 0; i < d->n_textures; i++) {
+        const swegl_texture_desc &t = d->textures[i];
+        HIPCHK(hipMemcpyAsync(s->texels.as<uint32_t>() + texs[i].offset, t.bitmap,
+                              (size_t)t.width * t.height * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if ((rc = scene_pose(c, s.get(), d)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));   // host vectors go out of scope
+    s->resident = V * (8 * 4 + 32) + tri.size() * 4 + texels * 4 +
+                  s->h_prims.size() * sizeof(PrimDev) + d->n_nodes * sizeof(NodeDev);
+    *out = s.release();
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_scene_update_pose(swegl_ctx *c, swegl_scene *s, const swegl_scene_desc *d)
+{
+    if (!c || !s || !d)
+        return fail(SWEGL_EINVAL, "null argument");
+    if (d->n_nodes != s->h_nodes.size())
+        return fail(SWEGL_EINVAL, "pose update changes the node count");
+    for (uint32_t i = 0; i < d->n_nodes; i++)
+        if (d->nodes[i].first_primitive != s->h_nodes[i].first_primitive ||
+            d->nodes[i].n_primitives != s->h_nodes[i].n_primitives)
+            return fail(SWEGL_EINVAL, "pose update changes primitives");
+    return scene_pose(c, s, d);
+}
+
+extern "C" int swegl_scene_destroy(swegl_ctx *c, swegl_scene *s)
+{
+    if (!s)
+        return SWEGL_OK;
+    if (c)
+        (void)hipStreamSynchronize(c->stream);
+    delete s;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_scene_bytes(swegl_ctx *c, swegl_scene *s, uint64_t *bytes)
+{
+    if (!c || !s || !bytes)
+        return fail(SWEGL_EINVAL, "null argument");
+    uint64_t b = s->resident;
+    for (auto &v : s->views)
+        b += v->ndc.n + v->yes.n + v->rect.n + v->big.n + v->dof.n;
+    *bytes = b;
+    return SWEGL_OK;
+}
+
+// ----------------------------------------------------------------- render
+struct JobIn {
+    swegl_scene *scene;
+    const swegl_viewport_desc *vp;
+};
+
+static int check_viewport(const swegl_viewport_desc &v)
+{
+    if (!v.color || v.color->kind != SWEGL_TARGET_COLOR)
+        return fail(SWEGL_EINVAL, "viewport colour target missing or not a colour surface");
+    if (v.w <= 0 || v.h <= 0 || v.x < 0 || v.y < 0 || v.x + v.w > v.color->w || v.y + v.h > v.color->h)
+        return fail(SWEGL_EINVAL, "viewport rect (%d,%d,%d,%d) outside the %dx%d surface", v.x, v.y, v.w,
+                    v.h, v.color->w, v.color->h);
+    if (v.w > 65535 * 8 || v.h > 65535 * 8)
+        return fail(SWEGL_EINVAL, "viewport too large");
+    if (!v.depth || v.depth->kind != SWEGL_TARGET_FLOAT || v.depth->w * v.depth->h != v.w * v.h)
+        return fail(SWEGL_EINVAL, "viewport depth target must hold w*h floats");
+    if (v.triangle_id && (v.triangle_id->kind != SWEGL_TARGET_UINT ||
+                          v.triangle_id->w * v.triangle_id->h != v.w * v.h))
+        return fail(SWEGL_EINVAL, "triangle id target must hold w*h uints");
+    if (v.transparency_layer_count > 0)
+        return fail(SWEGL_ENOTSUP, "transparency layers are not on the GPU path");
+    if (v.pixel_shader != SWEGL_SHADER_PHONG_BILINEAR)
+        return fail(SWEGL_ENOTSUP, "pixel shader %d is not on the GPU path", v.pixel_shader);
+    if (v.post_shader < SWEGL_POST_NULL || v.post_shader > SWEGL_POST_DEPTH_TAP9)
+        return fail(SWEGL_EINVAL, "post shader %d (the reference dereferences an unset post shader)",
+                    v.post_shader);
+    if (v.post_shader != SWEGL_POST_NULL &&
+        (v.dof_max_radius < 0 || v.dof_max_radius > SW_MAX_DOF_RADIUS))
+        return fail(SWEGL_EINVAL, "dof_max_radius %d outside [0,%d]", v.dof_max_radius, SW_MAX_DOF_RADIUS);
+    if (v.post_shader != SWEGL_POST_NULL &&
+        (!std::isfinite(v.focal_distance) || !std::isfinite(v.focal_depth)))
+        return fail(SWEGL_EINVAL, "focal distance and depth must be finite");
+    return SWEGL_OK;
+}
+
+static bool rects_overlap(const swegl_viewport_desc &a, const swegl_viewport_desc &b)
+{
+    return a.x < b.x + b.w && b.x < a.x + a.w && a.y < b.y + b.h && b.y < a.y + a.h;
+}
+static bool conflicts(const swegl_viewport_desc &a, const swegl_viewport_desc &b)
+{
+    if (a.color == b.color && rects_overlap(a, b))
+        return true;
+    if (a.depth == b.depth || (a.triangle_id && a.triangle_id == b.triangle_id))
+        return true;
+    return false;
+}
+
+static void m34_from(const float *m16, m34 &o) { memcpy(o.m, m16, 12 * sizeof(float)); }
+
+// Enqueue one wave of independent jobs.
+static int run_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_wave)
+{
+    const uint32_t nj = (uint32_t)jobs.size();
+    std::vector<JobDev> jd(nj);
+    std::map<swegl_scene *, int> scene_uses;
+    uint32_t vblk = 0, tblk = 0, dblk = 0, tiles = 0;
+    uint64_t tris_total = 0;
+    for (uint32_t j = 0; j < nj; j++) {
+        swegl_scene *s = jobs[j].scene;
+        const swegl_viewport_desc &v = *jobs[j].vp;
+        int use = scene_uses[s]++;
+        while ((int)s->views.size() <= use)
+            s->views.emplace_back(new ViewScratch);
+        ViewScratch &vs = *s->views[use];
+        const size_t V = std::max<uint32_t>(1, s->n_vertices), T = std::max<uint32_t>(1, s->n_tris);
+        int rc;
+        if ((rc = vs.ndc.ensure(V * 16)) || (rc = vs.yes.ensure(V)) || (rc = vs.rect.ensure(T * 8)) ||
+            (rc = vs.big.ensure(T * 4)))
+            return rc;
+        const bool dof = v.post_shader != SWEGL_POST_NULL;
+        if (dof && (rc = vs.dof.ensure((size_t)v.w * v.h * 4)))
+            return rc;
+        JobDev &J = jd[j];
+        memset(&J, 0, sizeof J);
+        J.px = s->px.as<float>();
+        J.py = s->py.as<float>();
+        J.pz = s->pz.as<float>();
+        J.nx = s->nx.as<float>();
+        J.ny = s->ny.as<float>();
+        J.nz = s->nz.as<float>();
+        J.tu = s->tu.as<float>();
+        J.tv = s->tv.as<float>();
+        J.tri = s->tri.as<uint32_t>();
+        J.prims = s->prims.as<PrimDev>();
+        J.nodes = s->nodes.as<NodeDev>();
+        J.texels = s->texels.as<uint32_t>();
+        J.texs = s->texs.as<TexDev>();
+        J.lights = s->lights.as<Light4>();
+        J.n_prims = s->n_prims;
+        J.n_lights = s->n_lights;
+        J.n_vertices = s->n_vertices;
+        J.n_tris = s->n_tris;
+        J.ambient = s->ambient;
+        J.sun = s->sun;
+        J.sun_I = s->sun_I;
+        J.world = s->world.as<float4>();
+        J.nworld = s->nworld.as<float4>();
+        J.write_world = (first_wave && use == 0) ? 1 : 0;
+        J.ndc = vs.ndc.as<float4>();
+        J.yes = vs.yes.as<uint8_t>();
+        J.rect = vs.rect.as<uint2>();
+        J.big = vs.big.as<uint32_t>();
+        m34_from(v.view, J.V.view);
+        m34_from(v.projection, J.V.proj);
+        hm::viewport_matrix(v.x, v.y, v.w, v.h, &J.V.m00, &J.V.m03, &J.V.m11, &J.V.m13);
+        J.V.vx = v.x;
+        J.V.vy = v.y;
+        J.V.vw = v.w;
+        J.V.vh = v.h;
+        J.V.cam = f3{v.camera_position[0], v.camera_position[1], v.camera_position[2]};
+        J.surface = v.color->buf.as<uint32_t>();
+        J.surface_pitch = v.color->w;
+        if (dof) {
+            J.out_color = vs.dof.as<uint32_t>();
+            J.out_pitch = v.w;
+            J.out_ox = v.x;
+            J.out_oy = v.y;
+        } else {
+            J.out_color = J.surface;
+            J.out_pitch = J.surface_pitch;
+            J.out_ox = 0;
+            J.out_oy = 0;
+        }
+        J.zbuf = v.depth->buf.as<float>();
+        J.tid = v.triangle_id ? v.triangle_id->buf.as<uint32_t>() : nullptr;
+        J.post = v.post_shader;
+        J.fdist = v.focal_distance;
+        J.fdepth = v.focal_depth;
+        J.rmax = v.dof_max_radius;
+        J.tw = c->tw;
+        J.th = c->th;
+        J.tiles_x = (v.w + c->tw - 1) / c->tw;
+        J.tiles_y = (v.h + c->th - 1) / c->th;
+        J.tile_base = tiles;
+        tiles += (uint32_t)(J.tiles_x * J.tiles_y);
+        J.vblock0 = vblk;
+        vblk += (s->n_vertices + 255) / 256;
+        J.tblock0 = tblk;
+        tblk += (s->n_tris + 255) / 256;
+        J.dblock0 = dblk;
+        if (dof)
+            dblk += (uint32_t)(((v.w + 15) / 16) * ((v.h + 15) / 16));
+        tris_total += s->n_tris;
+    }
+    int rc;
+    const uint32_t n_ctr = SW_CTR_BIG0 + nj;
+    const uint64_t cap = std::max<uint64_t>(1, tris_total * SW_BIG_TILES);
+    if (cap > 0xFFFFFFF0ull)
+        return fail(SWEGL_ERANGE, "too many triangles in one batch");
+    if ((rc = c->jobs.ensure(nj * sizeof(JobDev))) || (rc = c->tile_count.ensure((size_t)tiles * 4 + 4)) ||
+        (rc = c->tile_offset.ensure((size_t)tiles * 4 + 8)) || (rc = c->tile_cursor.ensure((size_t)tiles * 4 + 4)) ||
+        (rc = c->bins.ensure(cap * 4)) || (rc = c->ctr.ensure((size_t)n_ctr * 4)))
+        return rc;
+    if ((rc = upload(c, c->jobs.p, jd.data(), nj * sizeof(JobDev))))
+        return rc;
+    BatchDev B;
+    B.jobs = c->jobs.as<JobDev>();
+    B.n_jobs = nj;
+    B.n_tiles = tiles;
+    B.tile_count = c->tile_count.as<uint32_t>();
+    B.tile_offset = c->tile_offset.as<uint32_t>();
+    B.tile_cursor = c->tile_cursor.as<uint32_t>();
+    B.bins = c->bins.as<uint32_t>();
+    B.bin_capacity = (uint32_t)cap;
+    B.ctr = c->ctr.as<uint32_t>();
+    // stats words [0,16) are zeroed once per render by the first wave
+    BatchDev Bc = B;
+    if (!first_wave)
+        Bc.ctr = B.ctr + 16;
+    const uint32_t zero_words = first_wave ? n_ctr : n_ctr - 16;
+    hipStream_t st = c->stream;
+#define STAGE(name, call)                                                              \
+    do {                                                                               \
+        hipError_t e_ = timed(c, name, false, [&] { return call; });                    \
+        if (e_ != hipSuccess)                                                          \
+            return fail(SWEGL_EIO, "%s launch: %s", name, hipGetErrorString(e_));       \
+    } while (0)
+    STAGE("clear", launch_clear(Bc, zero_words, st));
+    STAGE("vertex", launch_vertex(B, vblk, st));
+    STAGE("mark", launch_mark(B, tblk, st));
+    STAGE("bin_count", launch_bin_count(B, tblk, st));
+    STAGE("scan", launch_scan(B, st));
+    STAGE("bin_fill", launch_bin_fill(B, tblk, st));
+    STAGE("raster", launch_raster(B, c->tw, c->th, st));
+    STAGE("dof", launch_dof(B, dblk, st));
+#undef STAGE
+    return SWEGL_OK;
+}
+
+static int render_jobs(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
+{
+    HIPCHK(hipSetDevice(c->device));
+    for (const JobIn &j : all) {
+        if (!j.scene)
+            return fail(SWEGL_EINVAL, "null scene");
+        int rc = check_viewport(*j.vp);
+        if (rc)
+            return rc;
+    }
+    // waves: a job conflicting with an earlier job of the current wave (same
+    // surface region or planes) starts a new wave, which keeps the
+    // reference's viewport order for overlapping viewports.
+    std::vector<std::vector<JobIn>> waves(1);
+    for (const JobIn &j : all) {
+        bool clash = false;
+        for (const JobIn &k : waves.back())
+            if (conflicts(*j.vp, *k.vp))
+                clash = true;
+        if (clash)
+            waves.emplace_back();
+        waves.back().push_back(j);
+    }
+    for (size_t w = 0; w < waves.size(); w++) {
+        if (waves[w].empty())
+            continue;
+        int rc = run_wave(c, waves[w], w == 0);
+        if (rc)
+            return rc;
+    }
+    if (stats) {
+        HIPCHK(hipMemcpyAsync(c->h_stats, c->ctr.p, 8 * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        drain_profile(c);
+        stats->fragments = c->h_stats[SW_ST_FRAGMENTS];
+        stats->nan_fragments = c->h_stats[SW_ST_NAN];
+        stats->drawn_subtriangles = c->h_stats[SW_ST_DRAWN];
+        stats->bin_entries = c->h_stats[SW_ST_BINS];
+        stats->texel_oob = c->h_stats[SW_ST_OOB];
+    }
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_render(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vps, uint32_t n,
+                            swegl_frame_stats *stats)
+{
+    if (!c || !s || (n && !vps))
+        return fail(SWEGL_EINVAL, "null argument");
+    std::vector<JobIn> jobs;
+    for (uint32_t i = 0; i < n; i++)
+        jobs.push_back({s, &vps[i]});
+    return render_jobs(c, jobs, stats);
+}
+
+extern "C" int swegl_render_graph(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vps,
+                                  uint32_t n, swegl_frame_stats *stats)
+{
+    return swegl_render(c, s, vps, n, stats);
+}
+
+extern "C" int swegl_render_batch(swegl_ctx *c, const swegl_job *jobs, uint32_t n, swegl_frame_stats *stats)
+{
+    if (!c || (n && !jobs))
+        return fail(SWEGL_EINVAL, "null argument");
+    std::vector<JobIn> all;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t k = 0; k < jobs[i].n_viewports; k++)
+            all.push_back({jobs[i].scene, &jobs[i].viewports[k]});
+    return render_jobs(c, all, stats);
+}
+
+// ----------------------------------------------------- host math exports
+extern "C" void swegl_matrix_rotate_x(float m[16], float a) { hm::rotate_x(m, a); }
+extern "C" void swegl_matrix_rotate_y(float m[16], float a) { hm::rotate_y(m, a); }
+extern "C" void swegl_matrix_rotate_z(float m[16], float a) { hm::rotate_z(m, a); }
+extern "C" void swegl_matrix_translate(float m[16], float x, float y, float z) { hm::translate(m, x, y, z); }
+extern "C" void swegl_camera_init(float aspect, float view[16], float proj[16], float center[3])
+{
+    hm::camera_init(aspect, view, proj, center);
+}
+extern "C" void swegl_camera_translate(float view[16], float center[3], float x, float y, float z)
+{
+    hm::camera_translate(view, center, x, y, z);
+}
+extern "C" void swegl_camera_rotate(float view[16], int axis, float a)
+{
+    if (axis == 0)
+        hm::rotate_x(view, -a);
+    else if (axis == 1)
+        hm::rotate_y(view, -a);
+    else
+        hm::rotate_z(view, -a);
+}

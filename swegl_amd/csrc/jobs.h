@@ -1,0 +1,107 @@
+// jobs.h -- device-side descriptors shared by the host orchestration
+// (capi.cpp) and the kernels (kernels.hip).
+//
+// HBM layout of a resident scene (SoA, all primitives concatenated in node
+// order, then primitive order):
+//   px,py,pz / nx,ny,nz / tu,tv : float[n_vertices]   mesh_vertex_t::v, normal, tex_coords
+//   tri                          : uint32[3*n_tris]    vertex ids per triangle in draw
+//                                                      order (strips/fans expanded with
+//                                                      the reference's winding)
+//   prims / nodes / texs         : small tables
+//   texels                       : uint32 bitmaps of all textures, concatenated
+//   world, nworld                : float4[n_vertices]  v_world / normal_world of the
+//                                                      current render (written once)
+// Per viewport job:
+//   ndc                          : float4[n_vertices]  frustum coordinates
+//   yes                          : uint8[n_vertices]   visibility marks
+//   rect                         : uint2[n_tris]       tile rectangle of each triangle
+//   big                          : uint32[n_tris]      triangles covering many tiles
+#pragma once
+#include <stdint.h>
+#include "core.h"
+#include "shade.h"
+
+namespace sw {
+
+struct PrimDev {
+    uint32_t first_vertex, n_vertices;
+    uint32_t first_tri, n_tris;
+    uint32_t node;
+    int32_t tex;           // texture index, or -1: 1x1 bitmap of `color`
+    uint32_t color;        // material colour (default bitmap, pixel_shaders.cpp:290-294)
+    int32_t double_sided;
+};
+
+struct NodeDev {
+    m34 M;     // original_to_world_matrix (rows 0..2)
+    m34 RS;    // scale(rotation, scale) for normals (vertex_shaders.hpp:63)
+};
+
+struct TexDev {
+    uint64_t offset;   // into texels
+    int32_t w, h;
+};
+
+#define SW_BIG_TILES 16      // triangles touching more tiles go to the per-job big list
+#define SW_MAX_DOF_RADIUS 16
+
+struct JobDev {
+    // ---- resident scene
+    const float *px, *py, *pz, *nx, *ny, *nz, *tu, *tv;
+    const uint32_t *tri;
+    const PrimDev *prims;
+    const NodeDev *nodes;
+    const uint32_t *texels;
+    const TexDev *texs;
+    const Light4 *lights;
+    uint32_t n_prims, n_lights, n_vertices, n_tris;
+    float ambient, sun_I;
+    f3 sun;
+    float4 *world, *nworld;
+    int32_t write_world;
+    // ---- per viewport
+    float4 *ndc;
+    uint8_t *yes;
+    uint2 *rect;
+    uint32_t *big;
+    ViewXf V;
+    uint32_t *out_color;      // raster colour target
+    int32_t out_pitch;        // words per row
+    int32_t out_ox, out_oy;   // absolute pixel (x,y) maps to out_color[(y-oy)*pitch + (x-ox)]
+    uint32_t *surface;        // final surface (SDL_Surface pixels)
+    int32_t surface_pitch;    // words per row
+    float *zbuf;              // m_zbuffer, w*h, viewport relative
+    uint32_t *tid;            // optional winning draw sequence, w*h
+    int32_t post;             // SWEGL_POST_*
+    float fdist, fdepth;
+    int32_t rmax;
+    // ---- tiles
+    int32_t tw, th, tiles_x, tiles_y;
+    uint32_t tile_base;       // first global tile id of this job
+    // ---- first block of this job in each batched launch
+    uint32_t vblock0, tblock0, dblock0;
+};
+
+struct BatchDev {
+    const JobDev *jobs;
+    uint32_t n_jobs;
+    uint32_t n_tiles;         // total tiles over all jobs
+    uint32_t *tile_count;     // [n_tiles]
+    uint32_t *tile_offset;    // [n_tiles + 1]
+    uint32_t *tile_cursor;    // [n_tiles]
+    uint32_t *bins;           // [capacity]
+    uint32_t bin_capacity;
+    uint32_t *ctr;            // counters: [0..15] stats as 8 u64, [16] raster queue,
+                              // [32 + j] big-list length of job j
+};
+
+// Counter words (uint64 pairs in ctr[0..15])
+#define SW_ST_FRAGMENTS 0
+#define SW_ST_NAN 1
+#define SW_ST_DRAWN 2
+#define SW_ST_BINS 3
+#define SW_ST_OOB 4
+#define SW_CTR_QUEUE 16
+#define SW_CTR_BIG0 32
+
+}  // namespace sw

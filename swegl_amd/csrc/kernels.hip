@@ -1,0 +1,744 @@
+// kernels.hip -- gfx950 kernels of the render path.
+//
+// Stage map (reference loop -> kernel):
+//   vertex_shaders.hpp:16-71 (v_world, normal_world, frustum coords)  -> k_vertex
+//   renderer.cpp:83-185 (visibility marks)                           -> k_mark
+//   renderer.cpp:240-460 (clip, DDA setup) for binning              -> k_bin_count / k_bin_fill
+//   renderer.cpp:462-558 (spans, z test) + pixel shading            -> k_raster (per tile)
+//   post_shaders.hpp:51-131 (depth of field)                        -> k_dof
+// Every kernel works on a batch of jobs (one job = one viewport of one
+// scene); a block finds its job by binary search over the job table, so the
+// plain path is simply a batch of one.
+#include <hip/hip_runtime.h>
+
+#include "jobs.h"
+#include "launch.h"
+
+namespace sw {
+
+// ------------------------------------------------------------ job lookup
+enum { JV = 0, JT = 1, JD = 2, JTILE = 3 };
+template <int F>
+__device__ __forceinline__ uint32_t first_block(const JobDev &j)
+{
+    return F == JV ? j.vblock0 : F == JT ? j.tblock0 : F == JD ? j.dblock0 : j.tile_base;
+}
+template <int F>
+__device__ __forceinline__ int find_job(const BatchDev &B, uint32_t blk)
+{
+    int lo = 0, hi = (int)B.n_jobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (first_block<F>(B.jobs[mid]) <= blk)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int prim_of_vertex(const JobDev &J, uint32_t v)
+{
+    int lo = 0, hi = (int)J.n_prims - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (J.prims[mid].first_vertex <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int prim_of_tri(const JobDev &J, uint32_t t)
+{
+    int lo = 0, hi = (int)J.n_prims - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (J.prims[mid].first_tri <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ f3 ld3(const float4 *a, uint32_t i)
+{
+    float4 v = a[i];
+    return {v.x, v.y, v.z};
+}
+
+// ------------------------------------------------------------- k_clear
+__global__ void k_clear(BatchDev B, uint32_t n_words)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < B.n_tiles)
+        B.tile_count[i] = 0;
+    if (i < n_words)
+        B.ctr[i] = 0;
+}
+
+// ------------------------------------------------------------ k_vertex
+// original_to_world (vertex_shaders.hpp:16-33) + world_to_camera_or_frustum
+// (:35-52, 61-71).  v_world is recomputed by every job from the same inputs
+// (bit-identical); only the job flagged write_world stores it.
+__global__ __launch_bounds__(256) void k_vertex(BatchDev B)
+{
+    const JobDev &J = B.jobs[find_job<JV>(B, blockIdx.x)];
+    uint32_t i = (blockIdx.x - J.vblock0) * 256u + threadIdx.x;
+    if (i >= J.n_vertices)
+        return;
+    const NodeDev &N = J.nodes[J.prims[prim_of_vertex(J, i)].node];
+    f3 w = xform(N.M, f3{J.px[i], J.py[i], J.pz[i]});
+    if (J.write_world) {
+        f3 n = normal_world(N.RS, f3{J.nx[i], J.ny[i], J.nz[i]});
+        J.world[i] = make_float4(w.x, w.y, w.z, 0.0f);
+        J.nworld[i] = make_float4(n.x, n.y, n.z, 0.0f);
+    }
+    f3 f = world_to_frustum(J.V, w);
+    J.ndc[i] = make_float4(f.x, f.y, f.z, 0.0f);
+    J.yes[i] = 0;
+}
+
+// -------------------------------------------------------------- k_mark
+// renderer.cpp:83-185: a vertex is marked when any triangle using it passes
+// the frustum test and (double-sided or front-facing) in frustum space.
+__global__ __launch_bounds__(256) void k_mark(BatchDev B)
+{
+    const JobDev &J = B.jobs[find_job<JT>(B, blockIdx.x)];
+    uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
+    if (t >= J.n_tris)
+        return;
+    uint32_t i0 = J.tri[3 * t], i1 = J.tri[3 * t + 1], i2 = J.tri[3 * t + 2];
+    f3 a = ld3(J.ndc, i0), b = ld3(J.ndc, i1), c = ld3(J.ndc, i2);
+    if (!inside_frustum(a, b, c))
+        return;
+    if (!J.prims[prim_of_tri(J, t)].double_sided && !(cross_nz(b - a, c - a) > 0))
+        return;
+    J.yes[i0] = 1;
+    J.yes[i1] = 1;
+    J.yes[i2] = 1;
+}
+
+// ------------------------------------------------ fill_triangle on device
+__device__ SVert orig_vertex(const JobDev &J, uint32_t i)
+{
+    SVert v;
+    v.s = frustum_to_screen(J.V, ld3(J.ndc, i));
+    v.a = v.b = i;
+    v.cut = 0.0f;
+    v.clip = 0;
+    return v;
+}
+// new vertex between a and b (renderer.cpp:290-297 and world_to_viewport)
+__device__ SVert clip_vertex(const JobDev &J, uint32_t a, uint32_t b, float cut)
+{
+    f3 wa = ld3(J.world, a), wb = ld3(J.world, b);
+    f3 w = wa + (wb - wa) * cut;
+    SVert v;
+    v.s = frustum_to_screen(J.V, world_to_frustum(J.V, w));
+    v.a = a;
+    v.b = b;
+    v.cut = cut;
+    v.clip = 1;
+    return v;
+}
+
+// fill_triangle (renderer.cpp:240-359): the sub-triangles handed to
+// fill_triangle_2, in call order.  Returns how many (0, 1 or 2).
+__device__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
+{
+    uint32_t i0 = J.tri[3 * t], i1 = J.tri[3 * t + 1], i2 = J.tri[3 * t + 2];
+    if (!(J.yes[i0] && J.yes[i1] && J.yes[i2]))
+        return 0;
+    SVert v0 = orig_vertex(J, i0), v1 = orig_vertex(J, i1), v2 = orig_vertex(J, i2);
+    bool front = cross_nz(v1.s - v0.s, v2.s - v0.s) < 0;
+    bool inv = false;
+    if (v1.s.z > v0.s.z) { SVert x = v0; v0 = v1; v1 = x; inv = !inv; }
+    if (v2.s.z > v1.s.z) { SVert x = v1; v1 = v2; v2 = x; inv = !inv; }
+    if (v1.s.z > v0.s.z) { SVert x = v0; v0 = v1; v1 = x; inv = !inv; }
+    if (v2.s.z >= SW_NEAR) {
+        out[0].v[0] = v0;
+        out[0].v[1] = v1;
+        out[0].v[2] = v2;
+        out[0].front = front;
+        return 1;
+    }
+    if (v1.s.z < SW_NEAR) {
+        float cut1 = (v0.s.z - 0.001f) / (v0.s.z - v1.s.z);
+        SVert n1 = clip_vertex(J, v0.a, v1.a, cut1);
+        float cut2 = (v0.s.z - 0.001f) / (v0.s.z - v2.s.z);
+        SVert n2 = clip_vertex(J, v0.a, v2.a, cut2);
+        bool f = cross_nz(n1.s - v0.s, n2.s - v0.s) < 0;
+        out[0].v[0] = v0;
+        out[0].v[1] = n1;
+        out[0].v[2] = n2;
+        out[0].front = inv ? !f : f;
+        return 1;
+    }
+    if (v2.s.z < SW_NEAR) {
+        float cut0 = (v0.s.z - 0.001f) / (v0.s.z - v2.s.z);
+        SVert n1 = clip_vertex(J, v0.a, v2.a, cut0);
+        float cut1 = (v1.s.z - 0.001f) / (v1.s.z - v2.s.z);
+        SVert n2 = clip_vertex(J, v1.a, v2.a, cut1);
+        bool f0 = cross_nz(v1.s - v0.s, n2.s - v0.s) < 0;
+        out[0].v[0] = v0;
+        out[0].v[1] = v1;
+        out[0].v[2] = n2;
+        out[0].front = inv ? !f0 : f0;
+        bool f1 = cross_nz(n2.s - v0.s, n1.s - v0.s) < 0;
+        out[1].v[0] = v0;
+        out[1].v[1] = n2;
+        out[1].v[2] = n1;
+        out[1].front = inv ? !f1 : f1;
+        return 2;
+    }
+    return 0;   // NaN depth: no branch of the reference is taken
+}
+
+// Rows a sub-triangle's halves process, and a conservative column range.
+struct Extent {
+    int y0, y1;   // [y0, y1)
+    int x0, x1;   // [x0, x1]
+};
+__device__ bool sub_extent(const Setup &S, const ViewXf &V, Extent &E)
+{
+    int ya = INT32_MAX, yb = INT32_MIN;
+    if (S.has_up && S.up_y1 > S.up_y0) {
+        ya = min(ya, S.up_y0);
+        yb = max(yb, S.up_y1);
+    }
+    if (S.has_lo && S.lo_y1 > S.lo_y0) {
+        ya = min(ya, S.lo_y0);
+        yb = max(yb, S.lo_y1);
+    }
+    if (ya >= yb)
+        return false;
+    E.y0 = ya;
+    E.y1 = yb;
+    float xmin = fminf(S.v[0].s.x, fminf(S.v[1].s.x, S.v[2].s.x));
+    float xmax = fmaxf(S.v[0].s.x, fmaxf(S.v[1].s.x, S.v[2].s.x));
+    // The DDA reaches row k after k float adds; each add is off by at most
+    // half an ulp of the largest |x| on the walk.  Widen by that bound.
+    float X = fmaxf(fmaxf(fabsf(xmin), fabsf(xmax)), 1.0f);
+    float slack = 2.0f + (float)(yb - ya + 8) * X * (1.0f / 1048576.0f);
+    if (!(xmin == xmin) || !(xmax == xmax) || X > 1e30f) {
+        E.x0 = V.vx;
+        E.x1 = V.vx + V.vw - 1;
+        return true;
+    }
+    float fx0 = floorf(xmin - slack), fx1 = ceilf(xmax + slack);
+    E.x0 = fx0 < (float)V.vx ? V.vx : (int)fx0;
+    E.x1 = fx1 > (float)(V.vx + V.vw - 1) ? V.vx + V.vw - 1 : (int)fx1;
+    return E.x0 <= E.x1;
+}
+
+#define RECT_EMPTY 0xFFFFFFFFu
+
+// ---------------------------------------------------------- k_bin_count
+__global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
+{
+    const int jidx = find_job<JT>(B, blockIdx.x);
+    const JobDev &J = B.jobs[jidx];
+    uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
+    uint32_t drawn = 0, entries = 0;
+    if (t < J.n_tris) {
+        SubTri st[2];
+        int ns = tri_subs(J, t, st);
+        int ya = INT32_MAX, yb = INT32_MIN, xa = INT32_MAX, xb = INT32_MIN;
+        for (int s = 0; s < ns; s++) {
+            Setup S;
+            if (!setup_subtri(st[s], J.V, S))
+                continue;
+            drawn++;
+            Extent E;
+            if (!sub_extent(S, J.V, E))
+                continue;
+            ya = min(ya, E.y0);
+            yb = max(yb, E.y1);
+            xa = min(xa, E.x0);
+            xb = max(xb, E.x1);
+        }
+        uint2 r = make_uint2(RECT_EMPTY, 0);
+        if (ya < yb && xa <= xb) {
+            int tx0 = (xa - J.V.vx) / J.tw, tx1 = (xb - J.V.vx) / J.tw;
+            int ty0 = (ya - J.V.vy) / J.th, ty1 = (yb - 1 - J.V.vy) / J.th;
+            r = make_uint2((uint32_t)tx0 | ((uint32_t)ty0 << 16), (uint32_t)tx1 | ((uint32_t)ty1 << 16));
+            int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+            if (n > SW_BIG_TILES) {
+                uint32_t pos = atomicAdd(&B.ctr[SW_CTR_BIG0 + jidx], 1u);
+                J.big[pos] = t;
+            } else {
+                for (int ty = ty0; ty <= ty1; ty++)
+                    for (int tx = tx0; tx <= tx1; tx++)
+                        atomicAdd(&B.tile_count[J.tile_base + ty * J.tiles_x + tx], 1u);
+                entries = n;
+            }
+        }
+        J.rect[t] = r;
+    }
+    // block totals for the frame statistics
+    __shared__ uint32_t s_drawn, s_entries;
+    if (threadIdx.x == 0) {
+        s_drawn = 0;
+        s_entries = 0;
+    }
+    __syncthreads();
+    if (drawn)
+        atomicAdd(&s_drawn, drawn);
+    if (entries)
+        atomicAdd(&s_entries, entries);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long *st = (unsigned long long *)B.ctr;
+        if (s_drawn)
+            atomicAdd(&st[SW_ST_DRAWN], (unsigned long long)s_drawn);
+        if (s_entries)
+            atomicAdd(&st[SW_ST_BINS], (unsigned long long)s_entries);
+    }
+}
+
+// ----------------------------------------------------------- k_scan
+// Exclusive scan of tile counts by one block of 1024 threads, each owning a
+// contiguous chunk.
+__global__ __launch_bounds__(1024) void k_scan(BatchDev B)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t n = B.n_tiles;
+    const uint32_t chunk = (n + 1023) / 1024;
+    const uint32_t a = threadIdx.x * chunk;
+    const uint32_t b = min(n, a + chunk);
+    uint32_t s = 0;
+    for (uint32_t i = a; i < b; i++)
+        s += B.tile_count[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (uint32_t i = a; i < b; i++) {
+        B.tile_offset[i] = run;
+        B.tile_cursor[i] = run;
+        run += B.tile_count[i];
+    }
+    if (threadIdx.x == 1023)
+        B.tile_offset[n] = part[1023];
+}
+
+// ----------------------------------------------------------- k_bin_fill
+__global__ __launch_bounds__(256) void k_bin_fill(BatchDev B)
+{
+    const JobDev &J = B.jobs[find_job<JT>(B, blockIdx.x)];
+    uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
+    if (t >= J.n_tris)
+        return;
+    uint2 r = J.rect[t];
+    if (r.x == RECT_EMPTY)
+        return;
+    int tx0 = r.x & 0xFFFF, ty0 = r.x >> 16, tx1 = r.y & 0xFFFF, ty1 = r.y >> 16;
+    if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > SW_BIG_TILES)
+        return;
+    for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1; tx++) {
+            uint32_t pos = atomicAdd(&B.tile_cursor[J.tile_base + ty * J.tiles_x + tx], 1u);
+            if (pos < B.bin_capacity)
+                B.bins[pos] = t;
+        }
+}
+
+// ------------------------------------------------------------- k_raster
+// One workgroup per screen tile.  Phase 1 replays every binned sub-triangle's
+// DDA over the tile's rows and columns (exact float sequence) and keeps the
+// minimum (z, draw sequence) per pixel with an LDS 64-bit atomic min.
+// Phase 2 shades each pixel's winner at its own progress and writes colour,
+// depth (and the winner's sequence) once, coalesced.
+struct TileRect { int x0, y0, x1, y1; };   // [x0,x1) x [y0,y1), absolute pixels
+
+__device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, unsigned long long *keys,
+                             int TW, uint32_t &frags, uint32_t &nans)
+{
+    SubTri st[2];
+    int ns = tri_subs(J, t, st);
+    for (int s = 0; s < ns; s++) {
+        Setup S;
+        if (!setup_subtri(st[s], J.V, S))
+            continue;
+        const uint32_t seq = 2u * t + (uint32_t)s;
+        int y = tr.y0;
+        while (y < tr.y1) {
+            bool lower;
+            int yend;
+            if (S.has_up && y >= S.up_y0 && y < S.up_y1) {
+                lower = false;
+                yend = min(S.up_y1, tr.y1);
+            } else if (S.has_lo && y >= S.lo_y0 && y < S.lo_y1) {
+                lower = true;
+                yend = min(S.lo_y1, tr.y1);
+            } else {
+                int nxt = tr.y1;
+                if (S.has_up && S.up_y0 > y && S.up_y1 > S.up_y0)
+                    nxt = min(nxt, S.up_y0);
+                if (S.has_lo && S.lo_y0 > y && S.lo_y1 > S.lo_y0)
+                    nxt = min(nxt, S.lo_y0);
+                y = nxt;
+                continue;
+            }
+            Row R;
+            row_state(S, y, R);
+            (void)lower;
+            for (; y < yend; y++) {
+                Span sp;
+                if (span_of(R, J.V, sp)) {
+                    int xa = max(sp.x1, tr.x0), xb = min(sp.x2, tr.x1);
+                    if (xa < xb) {
+                        Interp q = sp.q;
+                        q.steps(xa - sp.x1);
+                        unsigned long long *row = keys + (y - tr.y0) * TW - tr.x0;
+                        for (int x = xa; x < xb; x++, q.step()) {
+                            float z = q.value(q.u());
+                            if (z < SW_NEAR)
+                                continue;
+                            frags++;
+                            if (z != z) {
+                                nans++;
+                                continue;
+                            }
+                            if (__float_as_uint(z) >= SW_MAX_Z_BITS)
+                                continue;
+                            atomicMin(&row[x], vis_key(z, seq));
+                        }
+                    }
+                }
+                R.left.steps(1);
+                R.right.steps(1);
+            }
+        }
+    }
+}
+
+// Shading attributes of a sub-triangle vertex (v_world, normal_world, raw tex
+// coords); clip vertices are rebuilt as in renderer.cpp:290-297.
+__device__ void vertex_attr(const JobDev &J, const NodeDev &N, const SVert &v, f3 &w, f3 &n, f2 &t)
+{
+    if (!v.clip) {
+        w = ld3(J.world, v.a);
+        n = ld3(J.nworld, v.a);
+        t = f2{J.tu[v.a], J.tv[v.a]};
+        return;
+    }
+    f3 wa = ld3(J.world, v.a), wb = ld3(J.world, v.b);
+    w = wa + (wb - wa) * v.cut;
+    f2 ta = {J.tu[v.a], J.tv[v.a]}, tb = {J.tu[v.b], J.tv[v.b]};
+    t = ta + (tb - ta) * v.cut;
+    f3 na = {J.nx[v.a], J.ny[v.a], J.nz[v.a]}, nb = {J.nx[v.b], J.ny[v.b], J.nz[v.b]};
+    f3 nobj = (na.x == nb.x && na.y == nb.y && na.z == nb.z) ? na : normalize(na + (nb - na) * v.cut);
+    n = normal_world(N.RS, nobj);
+}
+
+__device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uint32_t &oob)
+{
+    const uint32_t t = seq >> 1, s = seq & 1u;
+    SubTri st[2];
+    Setup S;
+    Row R;
+    Span sp;
+    // The winner was produced by this very replay, so every step below
+    // succeeds; the guards only keep a corrupted key from looping.
+    if ((int)s >= tri_subs(J, t, st) || !setup_subtri(st[s], J.V, S) || !row_state(S, y, R) ||
+        !span_of(R, J.V, sp) || x < sp.x1 || x >= sp.x2)
+        return 0u;
+    Interp q = sp.q;
+    q.steps(x - sp.x1);
+    const float u = q.u();
+
+    const PrimDev &P = J.prims[prim_of_tri(J, t)];
+    const NodeDev &N = J.nodes[P.node];
+    uint32_t one = P.color;
+    const uint32_t *bm = &one;
+    int tw = 1, th = 1;
+    if (P.tex >= 0) {
+        TexDev T = J.texs[P.tex];
+        bm = J.texels + T.offset;
+        tw = T.w;
+        th = T.h;
+    }
+    TriAttr A;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        f3 w, n;
+        f2 tc;
+        vertex_attr(J, N, S.v[k], w, n, tc);
+        A.w[k] = w;
+        A.n[k] = S.inverted ? neg(n) : n;
+        A.t[k] = f2{tc.x * (float)tw, tc.y * (float)th};
+    }
+    ScanShade SS = prepare_scanline(A, R.lower, R.lr, sp.pl, sp.pr);
+    uint32_t color = bilinear(bm, tw, th, SS.t_left + SS.t_dir * u, oob);
+    f3 center = SS.v + SS.vdir * u;
+    f3 normal = normalize(SS.n + SS.ndir * u);
+    int32_t light = phong(center, normal, J.V.cam, J.ambient, J.sun, J.sun_I, J.lights, (int)J.n_lights);
+    return combine(color, light);
+}
+
+template <int TW, int TH>
+__global__ __launch_bounds__(256) void k_raster(BatchDev B)
+{
+    __shared__ unsigned long long keys[TW * TH];
+    __shared__ uint32_t s_cnt[3];
+    const uint32_t g = blockIdx.x;
+    const int jidx = find_job<JTILE>(B, g);
+    const JobDev &J = B.jobs[jidx];
+    const int lt = (int)(g - J.tile_base);
+    const int tx = lt % J.tiles_x, ty = lt / J.tiles_x;
+    TileRect tr;
+    tr.x0 = J.V.vx + tx * TW;
+    tr.y0 = J.V.vy + ty * TH;
+    tr.x1 = min(tr.x0 + TW, J.V.vx + J.V.vw);
+    tr.y1 = min(tr.y0 + TH, J.V.vy + J.V.vh);
+
+    for (int i = threadIdx.x; i < TW * TH; i += 256)
+        keys[i] = SW_KEY_EMPTY;
+    if (threadIdx.x < 3)
+        s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+
+    uint32_t frags = 0, nans = 0, oob = 0;
+    const uint32_t nb = B.tile_count[g], off = B.tile_offset[g];
+    const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
+    for (uint32_t e = threadIdx.x; e < nb + nbig; e += 256) {
+        uint32_t t;
+        if (e < nb) {
+            t = B.bins[off + e];
+        } else {
+            t = J.big[e - nb];
+            uint2 r = J.rect[t];
+            int rx0 = r.x & 0xFFFF, ry0 = r.x >> 16, rx1 = r.y & 0xFFFF, ry1 = r.y >> 16;
+            if (tx < rx0 || tx > rx1 || ty < ry0 || ty > ry1)
+                continue;
+        }
+        raster_depth(J, t, tr, keys, TW, frags, nans);
+    }
+    __syncthreads();
+
+    const float maxz = __uint_as_float(SW_MAX_Z_BITS);
+    for (int i = threadIdx.x; i < TW * TH; i += 256) {
+        const int x = tr.x0 + (i % TW), y = tr.y0 + (i / TW);
+        if (x >= tr.x1 || y >= tr.y1)
+            continue;
+        const unsigned long long k = keys[i];
+        uint32_t color = 0;
+        float z = maxz;
+        uint32_t seq = 0xFFFFFFFFu;
+        if (k != SW_KEY_EMPTY) {
+            seq = (uint32_t)k;
+            z = __uint_as_float((uint32_t)(k >> 32));
+            color = shade_pixel(J, seq, x, y, oob);
+        }
+        J.out_color[(int64_t)(y - J.out_oy) * J.out_pitch + (x - J.out_ox)] = color;
+        const int64_t zo = (int64_t)(y - J.V.vy) * J.V.vw + (x - J.V.vx);
+        J.zbuf[zo] = z;
+        if (J.tid)
+            J.tid[zo] = seq;
+    }
+
+    if (frags)
+        atomicAdd(&s_cnt[0], frags);
+    if (nans)
+        atomicAdd(&s_cnt[1], nans);
+    if (oob)
+        atomicAdd(&s_cnt[2], oob);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long *stc = (unsigned long long *)B.ctr;
+        if (s_cnt[0])
+            atomicAdd(&stc[SW_ST_FRAGMENTS], (unsigned long long)s_cnt[0]);
+        if (s_cnt[1])
+            atomicAdd(&stc[SW_ST_NAN], (unsigned long long)s_cnt[1]);
+        if (s_cnt[2])
+            atomicAdd(&stc[SW_ST_OOB], (unsigned long long)s_cnt[2]);
+    }
+}
+
+// ---------------------------------------------------------------- k_dof
+// Depth of field (build spec of post_shader_depth_box, post_shaders.hpp:51-131
+// with remap_clipped, lerp.hpp:24-43): 16x16 output pixels per block, colour
+// and blur factor of the tile plus an R_max halo staged in LDS.
+__device__ __forceinline__ float blur_factor(float z, float fdist, float fdepth, float rmax)
+{
+    float t = fabsf(fdist - z);
+    // inv_lerp_clipped(1, fdepth, t)
+    float x;
+    if (1.0f == fdepth)
+        x = 0.5f;
+    else if (t <= 1.0f)
+        x = 0.0f;
+    else if (t >= fdepth)
+        x = 1.0f;
+    else
+        x = (t - 1.0f) / (fdepth - 1.0f);
+    // lerp_clipped(0, rmax, x)
+    if (x <= 0)
+        return 0.0f;
+    if (x >= rmax)
+        return rmax;
+    return 0.0f + rmax * x;
+}
+
+#define DOF_T 16
+#define DOF_S (DOF_T + 2 * SW_MAX_DOF_RADIUS)
+__global__ __launch_bounds__(256) void k_dof(BatchDev B)
+{
+    __shared__ uint32_t s_col[DOF_S * DOF_S];
+    __shared__ float s_bf[DOF_S * DOF_S];
+    const JobDev &J = B.jobs[find_job<JD>(B, blockIdx.x)];
+    const int lb = (int)(blockIdx.x - J.dblock0);
+    const int bx = (J.V.vw + DOF_T - 1) / DOF_T;
+    const int ox = (lb % bx) * DOF_T, oy = (lb / bx) * DOF_T;   // viewport-relative
+    const int R = J.rmax;
+    const int S = DOF_T + 2 * R;
+    const float rmax = (float)R;
+    for (int i = threadIdx.x; i < S * S; i += 256) {
+        int lx = ox - R + i % S, ly = oy - R + i / S;
+        if (lx >= 0 && ly >= 0 && lx < J.V.vw && ly < J.V.vh) {
+            s_col[i] = J.out_color[(int64_t)(ly + J.V.vy - J.out_oy) * J.out_pitch + (lx + J.V.vx - J.out_ox)];
+            s_bf[i] = blur_factor(J.zbuf[(int64_t)ly * J.V.vw + lx], J.fdist, J.fdepth, rmax);
+        }
+    }
+    __syncthreads();
+    const int px = threadIdx.x % DOF_T, py = threadIdx.x / DOF_T;
+    const int x = ox + px, y = oy + py;
+    if (x >= J.V.vw || y >= J.V.vh)
+        return;
+    const int c = (py + R) * S + (px + R);
+    const uint32_t src = s_col[c];
+    const int radius = min(max(cvtt(s_bf[c]), 0), R);   // bf is in [0, R] for finite inputs
+    uint32_t out = src;
+    if (radius != 0) {
+        int b = 0, g = 0, r = 0, count = 0;
+        auto tap = [&](int i, int j) {
+            int k = (j - oy + R) * S + (i - ox + R);
+            if (s_bf[k] != 0) {
+                count++;
+                uint32_t p = s_col[k];
+                b += p & 0xFF;
+                g += (p >> 8) & 0xFF;
+                r += (p >> 16) & 0xFF;
+            }
+        };
+        if (J.post == 1) {   // box [x-r, x+r) x [y-r, y+r)
+            for (int j = max(0, y - radius); j < min(J.V.vh, y + radius); j++)
+                for (int i = max(0, x - radius); i < min(J.V.vw, x + radius); i++)
+                    tap(i, j);
+        } else {             // 3x3 taps at {-r, 0, r}
+            for (int a = -1; a <= 1; a++) {
+                int j = y + a * radius;
+                if (j < 0 || j >= J.V.vh)
+                    continue;
+                for (int e = -1; e <= 1; e++) {
+                    int i = x + e * radius;
+                    if (i < 0 || i >= J.V.vw)
+                        continue;
+                    tap(i, j);
+                }
+            }
+        }
+        if (count)
+            out = (uint32_t)(uint8_t)(b / count) | ((uint32_t)(uint8_t)(g / count) << 8) |
+                  ((uint32_t)(uint8_t)(r / count) << 16) | (255u << 24);
+    }
+    J.surface[(int64_t)(y + J.V.vy) * J.surface_pitch + (x + J.V.vx)] = out;
+}
+
+__global__ void k_fill32(uint32_t *p, uint64_t n, uint32_t w)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = w;
+}
+
+// ----------------------------------------------------------- launchers
+#define LAUNCH_CHECK(...)                                                      \
+    do {                                                                       \
+        hipLaunchKernelGGL(__VA_ARGS__);                                       \
+        hipError_t e_ = hipGetLastError();                                     \
+        if (e_ != hipSuccess)                                                  \
+            return e_;                                                         \
+    } while (0)
+
+hipError_t launch_clear(const BatchDev &B, uint32_t n_words, hipStream_t s)
+{
+    uint32_t n = B.n_tiles > n_words ? B.n_tiles : n_words;
+    LAUNCH_CHECK(k_clear, dim3((n + 255) / 256), dim3(256), 0, s, B, n_words);
+    return hipSuccess;
+}
+hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, hipStream_t s)
+{
+    if (blocks)
+        LAUNCH_CHECK(k_vertex, dim3(blocks), dim3(256), 0, s, B);
+    return hipSuccess;
+}
+hipError_t launch_mark(const BatchDev &B, uint32_t blocks, hipStream_t s)
+{
+    if (blocks)
+        LAUNCH_CHECK(k_mark, dim3(blocks), dim3(256), 0, s, B);
+    return hipSuccess;
+}
+hipError_t launch_bin_count(const BatchDev &B, uint32_t blocks, hipStream_t s)
+{
+    if (blocks)
+        LAUNCH_CHECK(k_bin_count, dim3(blocks), dim3(256), 0, s, B);
+    return hipSuccess;
+}
+hipError_t launch_scan(const BatchDev &B, hipStream_t s)
+{
+    LAUNCH_CHECK(k_scan, dim3(1), dim3(1024), 0, s, B);
+    return hipSuccess;
+}
+hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s)
+{
+    if (blocks)
+        LAUNCH_CHECK(k_bin_fill, dim3(blocks), dim3(256), 0, s, B);
+    return hipSuccess;
+}
+hipError_t launch_raster(const BatchDev &B, int tw, int th, hipStream_t s)
+{
+    if (!B.n_tiles)
+        return hipSuccess;
+    dim3 g(B.n_tiles), b(256);
+    if (tw == 8 && th == 8)
+        LAUNCH_CHECK((k_raster<8, 8>), g, b, 0, s, B);
+    else if (tw == 16 && th == 16)
+        LAUNCH_CHECK((k_raster<16, 16>), g, b, 0, s, B);
+    else if (tw == 32 && th == 16)
+        LAUNCH_CHECK((k_raster<32, 16>), g, b, 0, s, B);
+    else if (tw == 32 && th == 32)
+        LAUNCH_CHECK((k_raster<32, 32>), g, b, 0, s, B);
+    else if (tw == 64 && th == 32)
+        LAUNCH_CHECK((k_raster<64, 32>), g, b, 0, s, B);
+    else if (tw == 64 && th == 64)
+        LAUNCH_CHECK((k_raster<64, 64>), g, b, 0, s, B);
+    else
+        return hipErrorInvalidValue;
+    return hipSuccess;
+}
+hipError_t launch_dof(const BatchDev &B, uint32_t blocks, hipStream_t s)
+{
+    if (blocks)
+        LAUNCH_CHECK(k_dof, dim3(blocks), dim3(256), 0, s, B);
+    return hipSuccess;
+}
+
+hipError_t launch_fill32(uint32_t *p, uint64_t n, uint32_t word, hipStream_t s)
+{
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536)
+        blocks = 65536;
+    if (blocks)
+        LAUNCH_CHECK(k_fill32, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, word);
+    return hipSuccess;
+}
+
+}  // namespace sw

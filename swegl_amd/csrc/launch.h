@@ -1,0 +1,15 @@
+// launch.h -- host-side launchers of the kernels in kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "jobs.h"
+
+namespace sw {
+hipError_t launch_clear(const BatchDev &B, uint32_t n_words, hipStream_t s);
+hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, hipStream_t s);
+hipError_t launch_mark(const BatchDev &B, uint32_t blocks, hipStream_t s);
+hipError_t launch_bin_count(const BatchDev &B, uint32_t blocks, hipStream_t s);
+hipError_t launch_scan(const BatchDev &B, hipStream_t s);
+hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s);
+hipError_t launch_raster(const BatchDev &B, int tw, int th, hipStream_t s);
+hipError_t launch_dof(const BatchDev &B, uint32_t blocks, hipStream_t s);
+}  // namespace sw

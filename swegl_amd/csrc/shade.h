@@ -1,0 +1,216 @@
+// shade.h -- pixel_shader_light_and_texture<pixel_shader_lights_phong,
+// pixel_shader_texture_bilinear> for gfx950 (swegl/render/pixel_shaders.hpp,
+// src/render/pixel_shaders.cpp), same float operation order as the CPU.
+#pragma once
+#include "core.h"
+
+namespace sw {
+
+// (float)pow((double)s, 32) (pixel_shaders.cpp:193): s^32 in double-double
+// by five exact squarings (two-product through fma), rounded once to double
+// and then to float, as glibc pow followed by the float store does.
+SW_HD float pow32(float s)
+{
+    double hi = (double)s, lo = 0.0;
+    for (int i = 0; i < 5; i++) {
+        double p = hi * hi;
+        double e = fma(hi, hi, -p);
+        e += 2.0 * hi * lo;
+        double h2 = p + e;
+        lo = e - (h2 - p);
+        hi = h2;
+    }
+    return (float)(hi + lo);
+}
+
+struct Light4 { float x, y, z, I; };
+
+// pixel_shader_lights_phong::shade (pixel_shaders.cpp:159-205)
+SW_HD int32_t phong(f3 center, f3 normal, f3 cam, float ambient, f3 sun, float sun_I,
+                    const Light4 *lights, int n_lights)
+{
+    f3 camv = normalize(cam - center);
+    float fs = -dot(normal, sun);
+    if (fs < 0.0f)
+        fs = 0.0f;
+    else
+        fs *= sun_I;
+    float acc = 0.0f;
+    for (int i = 0; i < n_lights; i++) {
+        Light4 l = lights[i];
+        f3 L = center - f3{l.x, l.y, l.z};
+        float d2 = L.x * L.x + L.y * L.y + L.z * L.z;
+        float diffuse = l.I / d2;
+        if (diffuse < SW_DIFFUSE_CUT)
+            continue;
+        L = normalize(L);
+        float a = -dot(normal, L);
+        if (a < 0.0f)
+            continue;
+        diffuse *= a;
+        f3 R = L + normal * (a * 2.0f);
+        float s = dot(R, camv);
+        if (s > 0) {
+            s = pow32(s);
+            s = s * 32.0f / 2.0f;
+            acc += diffuse + s / d2;
+        } else {
+            acc += diffuse;
+        }
+    }
+    return cvtt(65536.0f * (ambient + fs + acc));
+}
+
+// pixel_shader_texture_bilinear::shade (pixel_shaders.cpp:348-384).  The
+// reference reads outside the bitmap for coordinates that wrap to a negative
+// index (UB); here such a tap is clamped and counted.
+SW_HD uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
+{
+    float v = t.x;
+    float u = t.y;
+    float u1 = (float)((double)u - 0.5);   // float - double literal, stored to float
+    float u2 = (float)((double)u + 0.5);
+    float v1 = (float)((double)v - 0.5);
+    float v2 = (float)((double)v + 0.5);
+    u = floorf(u2);
+    v = floorf(v2);
+    int v1m = wrap_add(cvtt(v1), th) % th;
+    int v2m = v1m + 1;
+    if (v2m == th)
+        v2m = 0;
+    v1m *= tw;
+    v2m *= tw;
+    int u1m = wrap_add(cvtt(u1), tw) % tw;
+    int u2m = u1m + 1;
+    if (u2m == tw)
+        u2m = 0;
+    const int64_t lim = (int64_t)tw * th;
+    int64_t idx[4] = {(int64_t)v1m + u1m, (int64_t)v2m + u1m, (int64_t)v1m + u2m,
+                      (int64_t)v2m + u2m};
+    uint32_t p[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int64_t i = idx[k];
+        if (i < 0 || i >= lim) {
+            oob++;
+            i = i < 0 ? 0 : lim - 1;
+        }
+        p[k] = pc[i];
+    }
+    const float w0 = (u - u1) * (v - v1);
+    const float w1 = (u - u1) * (v2 - v);
+    const float w2 = (u2 - u) * (v - v1);
+    const float w3 = (u2 - u) * (v2 - v);
+    uint32_t out = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float a = (float)((p[0] >> (8 * c)) & 0xFF) * w0;
+        float b = (float)((p[1] >> (8 * c)) & 0xFF) * w1;
+        float d = (float)((p[2] >> (8 * c)) & 0xFF) * w2;
+        float e = (float)((p[3] >> (8 * c)) & 0xFF) * w3;
+        float s = ((a + b) + d) + e;
+        out |= (uint32_t)to_u8(roundf(s)) << (8 * c);
+    }
+    return out;
+}
+
+// pixel_shader_light_and_texture::shade combine (pixel_shaders.hpp:159-178)
+SW_HD uint32_t combine(uint32_t color, int32_t light_i)
+{
+    float light = (float)((double)light_i / 65536.0);
+    uint32_t out = color & 0xFF000000u;
+    if (light < 1) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            uint32_t c = (color >> (8 * k)) & 0xFF;
+            out |= (uint32_t)to_u8((float)c * light) << (8 * k);
+        }
+    } else {
+        light = sqrtf(light);
+        light = sqrtf(light);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            uint32_t c = (color >> (8 * k)) & 0xFF;
+            uint8_t r = (uint8_t)(255 - to_u8((float)(255 - (int)c) / light));
+            out |= (uint32_t)r << (8 * k);
+        }
+    }
+    return out;
+}
+
+// Shader state of one triangle half and one scanline
+// (prepare_for_triangle / _upper / _lower / _scanline, pixel_shaders.cpp:88-158
+// and :304-346).
+struct TriAttr {
+    f3 w[3];    // v_world of the y-sorted vertices
+    f3 n[3];    // normal_world, negated when the triangle is inverted
+    f2 t[3];    // tex_coords * (width, height)
+};
+
+struct ScanShade {
+    f3 v, vdir, n, ndir;
+    f2 t_left, t_dir;
+};
+
+SW_HD ScanShade prepare_scanline(const TriAttr &A, bool lower, bool lr, float pl, float pr)
+{
+    f3 vleft, vright, vld, vrd, nleft, nright, nld, nrd;
+    f2 sst, sstd;
+    const f2 sltd = A.t[2] - A.t[0];
+    if (!lower) {
+        vleft = A.w[0];
+        vright = A.w[0];
+        nleft = A.n[0];
+        nright = A.n[0];
+        if (lr) {
+            vld = A.w[1] - A.w[0];
+            vrd = A.w[2] - A.w[0];
+            nld = A.n[1] - A.n[0];
+            nrd = A.n[2] - A.n[0];
+        } else {
+            vld = A.w[2] - A.w[0];
+            vrd = A.w[1] - A.w[0];
+            nld = A.n[2] - A.n[0];
+            nrd = A.n[1] - A.n[0];
+        }
+        sst = A.t[0];
+        sstd = A.t[1] - A.t[0];
+    } else {
+        if (lr) {
+            vright = A.w[0];
+            vrd = A.w[2] - A.w[0];
+            vleft = A.w[1];
+            vld = A.w[2] - A.w[1];
+            nright = A.n[0];
+            nrd = A.n[2] - A.n[0];
+            nleft = A.n[1];
+            nld = A.n[2] - A.n[1];
+        } else {
+            vleft = A.w[0];
+            vld = A.w[2] - A.w[0];
+            vright = A.w[1];
+            vrd = A.w[2] - A.w[1];
+            nleft = A.n[0];
+            nld = A.n[2] - A.n[0];
+            nright = A.n[1];
+            nrd = A.n[2] - A.n[1];
+        }
+        sst = A.t[1];
+        sstd = A.t[2] - A.t[1];
+    }
+    ScanShade S;
+    S.v = vleft + vld * pl;
+    S.vdir = vright + vrd * pr - S.v;
+    S.n = nleft + nld * pl;
+    S.ndir = nright + nrd * pr - S.n;
+    if (lr) {
+        S.t_left = sst + sstd * pl;
+        S.t_dir = A.t[0] + sltd * pr - S.t_left;
+    } else {
+        S.t_left = A.t[0] + sltd * pl;
+        S.t_dir = sst + sstd * pr - S.t_left;
+    }
+    return S;
+}
+
+}  // namespace sw

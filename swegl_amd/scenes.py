@@ -1,0 +1,215 @@
+"""Seeded synthetic scenes for the benchmark workloads.
+
+Every generator is deterministic in its seed: numpy's PCG64 bit stream and
+IEEE +,-,*,/,sqrt,floor only (no transcendental functions except through
+the library's own host helpers, which use the C library's cosf/sinf).
+
+Workloads
+  cube      : make_cube(1, 0) rotated by seeded Euler angles, 256^2 texture,
+              one sun, 640x480, no depth of field.
+  heightfield: n x n grid on [-1,1]^2 with 4-octave value noise (amplitude
+              0.2), finite-difference normals, 2048^2 texture, sun + 4 point
+              lights, 1920x1080, 9-tap depth of field (r_max 8).
+  batch     : 256 height fields, T log-uniform in [1e3, 2e5], square
+              framebuffers, 512^2/1024^2 textures, 0-4 point lights, DoF on
+              for half of them.
+"""
+import math
+
+import numpy as np
+
+from . import _abi as A
+from .host import (IDENTITY, Surface, material_t, node_t, pixel_colors,
+                   pixel_shader_light_and_texture, point_source_light,
+                   post_shader_depth_box, post_shader_t, primitive_t, rotate_x,
+                   rotate_y, rotate_z, scene_t, texture_t, viewport_t)
+
+
+def f32(x):
+    return np.float32(x)
+
+
+def normal_t(x, y, z):
+    """normal_t(x,y,z): normalised with the reference's float steps."""
+    x, y, z = f32(x), f32(y), f32(z)
+    l = np.sqrt(f32(f32(x * x) + f32(y * y)) + f32(z * z), dtype=np.float32)
+    if l != 0:
+        x, y, z = x / l, y / l, z / l
+    return np.array([x, y, z], np.float32)
+
+
+# --------------------------------------------------------------- make_cube
+_CUBE = [  # (positions sign, tex, normal) per face: model.hpp:254-325
+    ([(-1, 1, 1), (1, 1, 1), (1, 1, -1), (-1, 1, -1)],
+     [(0.0, 0.0), (0.5, 0.0), (0.5, 0.333), (0.0, 0.333)], (0, 1, 0)),
+    ([(-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)],
+     [(0.5, 0.0), (1.0, 0.0), (1.0, 0.333), (0.5, 0.333)], (0, 0, 1)),
+    ([(1, -1, 1), (1, -1, -1), (1, 1, -1), (1, 1, 1)],
+     [(0.0, 0.333), (0.5, 0.333), (0.5, 0.667), (0.0, 0.667)], (1, 0, 0)),
+    ([(1, -1, -1), (-1, -1, -1), (-1, 1, -1), (1, 1, -1)],
+     [(0.5, 0.667), (1.0, 0.667), (1.0, 1.0), (0.5, 1.0)], (0, 0, -1)),
+    ([(-1, -1, -1), (-1, -1, 1), (-1, 1, 1), (-1, 1, -1)],
+     [(0.5, 0.333), (1.0, 0.333), (1.0, 0.667), (0.5, 0.667)], (-1, 0, 0)),
+    ([(-1, -1, -1), (1, -1, -1), (1, -1, 1), (-1, -1, 1)],
+     [(0.0, 0.667), (0.5, 0.667), (0.5, 1.0), (0.0, 1.0)], (0, -1, 0)),
+]
+
+
+def make_cube(size, material_idx):
+    """make_cube (swegl/data/model.hpp:254-325): six 4-vertex fans."""
+    h = f32(size) / f32(2.0)
+    node = node_t()
+    for signs, tex, n in _CUBE:
+        pos = np.array([[s * h for s in p] for p in signs], np.float32)
+        nor = np.tile(normal_t(*n), (4, 1))
+        uv = np.array(tex, np.float32)   # double literals stored to float
+        node.primitives.append(primitive_t(pos, nor, uv, [0, 1, 2, 3], A.TRIANGLE_FAN,
+                                           material_idx))
+    return node
+
+
+def random_texture(rng, w, h):
+    """w x h pixel_colors with every byte i.i.d. uniform in [0, 255]."""
+    return texture_t(rng.integers(0, 1 << 32, size=(h, w), dtype=np.uint64).astype(np.uint32))
+
+
+def set_fov(camera, aspect, fov_deg=60.0):
+    """Horizontal FOV through the public projection matrix, keeping
+    P11 = aspect * P00 (camera.cpp:21-24 convention)."""
+    p00 = f32(1.0 / math.tan(math.radians(fov_deg) / 2.0))
+    camera.m_projectionmatrix[0, 0] = p00
+    camera.m_projectionmatrix[1, 1] = f32(f32(aspect) * p00)
+
+
+def seeded_sun(rng):
+    v = rng.standard_normal(3).astype(np.float32)
+    v[1] = -abs(v[1]) - f32(0.5)   # mostly from above
+    return normal_t(*v)
+
+
+# ------------------------------------------------------------------- cube
+def cube_scene(seed=0, width=640, height=480):
+    """Workload 'cube': returns (scene, [viewport])."""
+    rng = np.random.default_rng(seed)
+    a, b, c = (float(f32(x)) for x in rng.uniform(0.0, 2.0 * math.pi, 3))
+    s = scene_t()
+    s.images = [random_texture(rng, 256, 256)]
+    s.materials = [material_t(pixel_colors(255, 255, 255, 255), 1, 1, 0, False)]
+    cube = make_cube(1.0, 0)
+    R = IDENTITY.copy()
+    R = rotate_x(R, a)
+    R = rotate_y(R, b)
+    R = rotate_z(R, c)
+    cube.rotation = R
+    s.nodes = [cube]
+    s.root_nodes = [0]
+    s.ambient_light_intensity = float(f32(0.1))
+    s.sun_direction = seeded_sun(rng)
+    s.sun_intensity = 1.0
+    surface = Surface(width, height)
+    vp = viewport_t(0, 0, width, height, surface, pixel_shader_light_and_texture(), 0)
+    vp.m_camera.translate(0, 0, -3)
+    set_fov(vp.m_camera, f32(1.0 * width / height))
+    vp.set_post_shader(post_shader_t())
+    return s, [vp]
+
+
+# ------------------------------------------------------------ height field
+def value_noise(n, rng, octaves=4, base=4):
+    """Sum of `octaves` bilinear-smoothstep value-noise layers on an n x n
+    grid over [0,1]^2, frequencies base*2^k, weights 2^-k, normalised."""
+    g = np.arange(n, dtype=np.float64) / (n - 1)
+    total = np.zeros((n, n), np.float64)
+    wsum = 0.0
+    for k in range(octaves):
+        f = base << k
+        lat = rng.uniform(-1.0, 1.0, (f + 1, f + 1))
+        x = g * f
+        i = np.minimum(np.floor(x).astype(np.int64), f - 1)
+        t = x - i
+        s = t * t * (3.0 - 2.0 * t)
+        a = lat[i][:, i]
+        b = lat[i + 1][:, i]
+        c = lat[i][:, i + 1]
+        d = lat[i + 1][:, i + 1]
+        sx = s[:, None]
+        sz = s[None, :]
+        layer = (a * (1 - sx) + b * sx) * (1 - sz) + (c * (1 - sx) + d * sx) * sz
+        w = 0.5 ** k
+        total += w * layer
+        wsum += w
+    return total / wsum
+
+
+def heightfield_mesh(n, rng, amplitude=0.2, material=0):
+    """n x n vertex grid, x and z in [-1,1], y = amplitude * noise.
+    Triangles (a, c, b) and (a, d, c) per cell, counter-clockwise from +y."""
+    h = amplitude * value_noise(n, rng)
+    coord = np.linspace(-1.0, 1.0, n)
+    X, Z = np.meshgrid(coord, coord, indexing="ij")   # [i, j]: x = coord[i], z = coord[j]
+    pos = np.stack([X, h, Z], -1).reshape(-1, 3).astype(np.float32)
+    dx = np.gradient(h, coord, axis=0)
+    dz = np.gradient(h, coord, axis=1)
+    nor = np.stack([-dx, np.ones_like(h), -dz], -1)
+    nor /= np.sqrt((nor * nor).sum(-1, keepdims=True))
+    nor = nor.reshape(-1, 3).astype(np.float32)
+    gi = np.arange(n, dtype=np.float64) / (n - 1)
+    U, Vv = np.meshgrid(gi, gi, indexing="ij")
+    uv = np.stack([U, Vv], -1).reshape(-1, 2).astype(np.float32)
+    i = np.arange(n - 1)
+    I, J = np.meshgrid(i, i, indexing="ij")
+    a = (I * n + J).reshape(-1)
+    b = ((I + 1) * n + J).reshape(-1)
+    c = ((I + 1) * n + J + 1).reshape(-1)
+    d = (I * n + J + 1).reshape(-1)
+    tris = np.stack([np.stack([a, c, b], -1), np.stack([a, d, c], -1)], 1).reshape(-1)
+    node = node_t([primitive_t(pos, nor, uv, tris.astype(np.uint32), A.TRIANGLES, material)])
+    return node
+
+
+def heightfield_scene(seed=0, n=708, width=1920, height=1080, tex=2048, n_lights=4, dof=True,
+                      dof_radius=8, light_intensity=(0.5, 2.0)):
+    """Workload 'heightfield' (n=708 gives 999,698 triangles)."""
+    rng = np.random.default_rng(seed)
+    s = scene_t()
+    s.images = [random_texture(rng, tex, tex)]
+    s.materials = [material_t(pixel_colors(255, 255, 255, 255), 1, 1, 0, False)]
+    s.nodes = [heightfield_mesh(n, rng)]
+    s.root_nodes = [0]
+    s.ambient_light_intensity = float(f32(0.1))
+    s.sun_direction = seeded_sun(rng)
+    s.sun_intensity = float(f32(0.7))
+    for _ in range(n_lights):
+        p = rng.uniform(-2.0, 2.0, 3).astype(np.float32)
+        I = f32(rng.uniform(*light_intensity))
+        s.point_source_lights.append(point_source_light(p, I))
+    surface = Surface(width, height)
+    vp = viewport_t(0, 0, width, height, surface, pixel_shader_light_and_texture(), 0)
+    cam = vp.m_camera
+    cam.translate(0.0, 1.1, -1.9)
+    cam.rotate_x(-0.55)
+    set_fov(cam, f32(1.0 * width / height))
+    if dof:
+        vp.set_post_shader(post_shader_depth_box(5.0, 5.0, vp, max_radius=dof_radius, taps="tap9"))
+    else:
+        vp.set_post_shader(post_shader_t())
+    return s, [vp]
+
+
+def batch_scenes(seed=0, count=256, tmin=1_000, tmax=200_000,
+                 sides=(256, 384, 512, 768, 1024)):
+    """Workload 'batch': list of (scene, [viewport]) pairs."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        t = math.exp(rng.uniform(math.log(tmin), math.log(tmax)))
+        n = int(round(math.sqrt(t / 2.0))) + 1
+        side = int(sides[rng.integers(0, len(sides))])
+        tex = 512 if rng.integers(0, 2) == 0 else 1024
+        nl = int(rng.integers(0, 5))
+        dof = (k % 2) == 0
+        sub_seed = int(rng.integers(0, 1 << 31))
+        s, vps = heightfield_scene(sub_seed, n=n, width=side, height=side, tex=tex, n_lights=nl,
+                                   dof=dof)
+        out.append((s, vps))
+    return out
