@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X render path.
+
+Default workload (one GPU): the 999,698-triangle height field (708x708 grid,
+2048^2 texture, sun + 4 point lights) rendered to a 1920x1080 viewport with
+the 9-tap depth of field, scene resident in HBM.  One step = one frame of
+the plain path (vertex, marking, binning, tile raster + shading, DoF).
+
+Prints ONE JSON line.  Extra fields: Gfragments/s, p50/p99 frame latency,
+per-kernel averages, the roofline of the dominant kernel (HIP events on the
+library stream) and the CPU oracle timed on this host on a bounded sample.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "frames/s + Gfragments/s, batched vs plain on GPU; p50/p99 frame latency"
+
+
+def algorithmic_bytes(scene, vp, f_vis):
+    """Bytes a frame must move at minimum: vertex inputs (pos+normal+uv,
+    32 B), indices (4 B), texels touched (bounded by 16 B per shaded pixel),
+    colour+depth written (8 B/px) and the DoF pass (12 B/px)."""
+    V = scene.n_vertices()
+    I = sum(len(p.indices) for n in scene.nodes for p in n.primitives)
+    tex = sum(t.width * t.height * 4 for t in scene.images)
+    P = vp.m_w * vp.m_h
+    dof = vp.m_post_shader is not None and vp.m_post_shader.kind != 0
+    frame = V * 32 + I * 4 + min(tex, 16 * f_vis) + P * 8 + (P * 12 if dof else 0)
+    raster = min(tex, 16 * f_vis) + P * 8
+    return frame, raster
+
+
+def cpu_baseline(scene, vps, budget_s):
+    import oracle
+    oracle.render(scene, vps)          # warm (page in, build caches)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle.render(scene, vps)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 200:
+            break
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": n / el, "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"{n} frames of the same workload, single thread, {el:.1f} s ({cpu})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tile", default=None, help="WxH raster tile, e.g. 32x32")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the render path is single-GPU in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from swegl_amd import _abi as A
+    from swegl_amd import scenes
+    from swegl_amd.device import Context, DeviceScene, Target, check, lib
+    from swegl_amd.host import SceneDescHolder
+
+    scene, vps = scenes.heightfield_scene(args.seed)
+    vp = vps[0]
+    ctx = Context(0)
+    if args.tile:
+        tw, th = (int(x) for x in args.tile.lower().split("x"))
+        ctx.set_tile(tw, th)
+    info = ctx.info()
+    ds = DeviceScene(ctx, SceneDescHolder(scene))
+    surf = Target(ctx, A.TARGET_COLOR, vp.m_screen.w, vp.m_screen.h)
+    zt = Target(ctx, A.TARGET_FLOAT, vp.m_w, vp.m_h)
+    desc = (A.ViewportDesc * 1)()
+    desc[0] = vp.desc(surf, zt)
+    L = lib()
+
+    def frame(stats=None):
+        check(L.swegl_render(ctx.h, ds.h, desc, 1, stats), "swegl_render")
+
+    st = (A.FrameStats * 1)()
+    frame(st)
+    stats = {k: getattr(st[0], k) for k, _ in A.FrameStats._fields_}
+    for _ in range(args.warmup):
+        frame()
+    ctx.sync()
+
+    # timed region: exactly K frames, synchronised on both sides
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    fps = 1e3 / ms
+
+    # per-frame latency (submit -> complete), frames one at a time
+    lat = []
+    for _ in range(max(20, min(args.steps, 200))):
+        a = time.perf_counter()
+        frame()
+        ctx.sync()
+        lat.append((time.perf_counter() - a) * 1e3)
+    lat = np.array(lat)
+
+    # per-kernel HIP-event timing on the library stream
+    ctx.set_profiling(True)
+    for _ in range(10):
+        frame()
+    ctx.sync()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    dom = max(kt.items(), key=lambda kv: kv[1][0])
+    f_vis = None
+    # shaded pixels of this frame (for the texel term)
+    z = zt.read()
+    f_vis = int((z.view(np.uint32) != 0x7F7F7F7F).sum())
+    frame_bytes, raster_bytes = algorithmic_bytes(scene, vp, f_vis)
+    dom_bytes = raster_bytes if dom[0] == "raster" else None
+    achieved = dom_bytes / (dom[1][0] * 1e-3) / 1e9 if dom_bytes else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_heightfield_1080p.json")
+    if os.path.exists(tfile):
+        try:
+            with open(tfile) as f:
+                tj = json.load(f)
+            traffic = tj.get(dom[0])
+        except (OSError, ValueError):
+            traffic = None
+
+    out = {
+        "metric": METRIC,
+        "value": fps,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded height field, random texture)",
+        "config": {"workload": "heightfield 999,698 triangles, 1920x1080, tex 2048^2, "
+                               "sun + 4 point lights, 9-tap DoF r_max 8",
+                   "path": "plain", "tile": "%dx%d" % ctx.get_tile(), "seed": args.seed},
+        "gfragments_per_s": stats["fragments"] * fps / 1e9,
+        "fragments_per_frame": stats["fragments"],
+        "shaded_pixels": f_vis,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "frame_algorithmic_bytes": frame_bytes,
+        "frame_hbm_frac": frame_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()},
+        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": traffic, "algorithmic_bytes": dom_bytes},
+        "device": info,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, vps, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -67,6 +67,32 @@ __device__ __forceinline__ f3 ld3(const float4 *a, uint32_t i)
     return {v.x, v.y, v.z};
 }
 
+// Wave-aggregated atomicAdd: lanes holding the same key share one atomic
+// (ballot / readlane per distinct key).  Returns this lane's slot
+// (old value + rank among same-key lanes); inactive lanes get 0.
+__device__ __forceinline__ uint32_t wave_add(uint32_t *base, uint32_t key, bool active)
+{
+    const int lane = __lane_id();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint64_t pending = __ballot(active);
+    uint32_t slot = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lkey = __shfl(key, leader);
+        const uint64_t same = __ballot(active && key == lkey);
+        uint32_t old = 0;
+        if (lane == leader)
+            old = atomicAdd(&base[lkey], (uint32_t)__popcll(same));
+        old = __shfl(old, leader);
+        if (active && key == lkey) {
+            slot = old + (uint32_t)__popcll(same & lt);
+            active = false;
+        }
+        pending &= ~same;
+    }
+    return slot;
+}
+
 // ------------------------------------------------------------- k_clear
 __global__ void k_clear(BatchDev B, uint32_t n_words)
 {
@@ -241,13 +267,15 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
     const JobDev &J = B.jobs[jidx];
     uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
     uint32_t drawn = 0, entries = 0;
+    int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0;
     if (t < J.n_tris) {
         SubTri st[2];
         int ns = tri_subs(J, t, st);
         int ya = INT32_MAX, yb = INT32_MIN, xa = INT32_MAX, xb = INT32_MIN;
-        for (int s = 0; s < ns; s++) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
             Setup S;
-            if (!setup_subtri(st[s], J.V, S))
+            if (s >= ns || !setup_subtri(st[s], J.V, S))
                 continue;
             drawn++;
             Extent E;
@@ -260,21 +288,30 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
         }
         uint2 r = make_uint2(RECT_EMPTY, 0);
         if (ya < yb && xa <= xb) {
-            int tx0 = (xa - J.V.vx) / J.tw, tx1 = (xb - J.V.vx) / J.tw;
-            int ty0 = (ya - J.V.vy) / J.th, ty1 = (yb - 1 - J.V.vy) / J.th;
+            tx0 = (xa - J.V.vx) / J.tw;
+            tx1 = (xb - J.V.vx) / J.tw;
+            ty0 = (ya - J.V.vy) / J.th;
+            ty1 = (yb - 1 - J.V.vy) / J.th;
             r = make_uint2((uint32_t)tx0 | ((uint32_t)ty0 << 16), (uint32_t)tx1 | ((uint32_t)ty1 << 16));
-            int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-            if (n > SW_BIG_TILES) {
+            ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+            if (ntiles > SW_BIG_TILES) {
                 uint32_t pos = atomicAdd(&B.ctr[SW_CTR_BIG0 + jidx], 1u);
                 J.big[pos] = t;
-            } else {
-                for (int ty = ty0; ty <= ty1; ty++)
-                    for (int tx = tx0; tx <= tx1; tx++)
-                        atomicAdd(&B.tile_count[J.tile_base + ty * J.tiles_x + tx], 1u);
-                entries = n;
+                ntiles = 0;
             }
+            entries = ntiles;
         }
         J.rect[t] = r;
+    }
+    // one aggregated atomic per distinct tile per wave and round
+    for (int k = 0; __any(k < ntiles); k++) {
+        bool act = k < ntiles;
+        uint32_t key = 0;
+        if (act) {
+            int w = tx1 - tx0 + 1;
+            key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
+        }
+        wave_add(B.tile_count, key, act);
     }
     // block totals for the frame statistics
     __shared__ uint32_t s_drawn, s_entries;
@@ -333,20 +370,30 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B)
 {
     const JobDev &J = B.jobs[find_job<JT>(B, blockIdx.x)];
     uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
-    if (t >= J.n_tris)
-        return;
-    uint2 r = J.rect[t];
-    if (r.x == RECT_EMPTY)
-        return;
-    int tx0 = r.x & 0xFFFF, ty0 = r.x >> 16, tx1 = r.y & 0xFFFF, ty1 = r.y >> 16;
-    if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > SW_BIG_TILES)
-        return;
-    for (int ty = ty0; ty <= ty1; ty++)
-        for (int tx = tx0; tx <= tx1; tx++) {
-            uint32_t pos = atomicAdd(&B.tile_cursor[J.tile_base + ty * J.tiles_x + tx], 1u);
-            if (pos < B.bin_capacity)
-                B.bins[pos] = t;
+    int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
+    if (t < J.n_tris) {
+        uint2 r = J.rect[t];
+        if (r.x != RECT_EMPTY) {
+            tx0 = r.x & 0xFFFF;
+            ty0 = r.x >> 16;
+            tx1 = r.y & 0xFFFF;
+            int ty1 = r.y >> 16;
+            ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+            if (ntiles > SW_BIG_TILES)
+                ntiles = 0;
         }
+    }
+    for (int k = 0; __any(k < ntiles); k++) {
+        bool act = k < ntiles;
+        uint32_t key = 0;
+        if (act) {
+            int w = tx1 - tx0 + 1;
+            key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
+        }
+        uint32_t pos = wave_add(B.tile_cursor, key, act);
+        if (act && pos < B.bin_capacity)
+            B.bins[pos] = t;
+    }
 }
 
 // ------------------------------------------------------------- k_raster
@@ -362,9 +409,10 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, unsigned 
 {
     SubTri st[2];
     int ns = tri_subs(J, t, st);
-    for (int s = 0; s < ns; s++) {
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
         Setup S;
-        if (!setup_subtri(st[s], J.V, S))
+        if (s >= ns || !setup_subtri(st[s], J.V, S))
             continue;
         const uint32_t seq = 2u * t + (uint32_t)s;
         int y = tr.y0;
@@ -447,7 +495,7 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
     Span sp;
     // The winner was produced by this very replay, so every step below
     // succeeds; the guards only keep a corrupted key from looping.
-    if ((int)s >= tri_subs(J, t, st) || !setup_subtri(st[s], J.V, S) || !row_state(S, y, R) ||
+    if ((int)s >= tri_subs(J, t, st) || !setup_subtri(s ? st[1] : st[0], J.V, S) || !row_state(S, y, R) ||
         !span_of(R, J.V, sp) || x < sp.x1 || x >= sp.x2)
         return 0u;
     Interp q = sp.q;
