@@ -157,6 +157,8 @@ typedef struct swegl_frame_stats {
     uint64_t bin_entries;        /* (triangle, tile) pairs                        */
     uint64_t texel_oob;          /* bilinear taps outside the bitmap (UB in the
                                     reference; clamped here)                      */
+    uint64_t big_triangles;      /* triangles over the bin limit (tested by every
+                                    tile of their viewport)                       */
 } swegl_frame_stats;
 
 /* ---- context ----------------------------------------------------------- */
@@ -172,6 +174,10 @@ int  swegl_device_info(swegl_ctx *ctx, int *n_cus, char *arch, int arch_cap,
  * capacity hint; the autotuner writes these. */
 int  swegl_ctx_set_tile(swegl_ctx *ctx, int tile_w, int tile_h);
 int  swegl_ctx_get_tile(swegl_ctx *ctx, int *tile_w, int *tile_h);
+/* Bin limit: a triangle whose tile rectangle holds more than `tiles` tiles
+ * is not binned per tile but listed once per viewport and tested by every
+ * tile (default 16; 0 sends every triangle through that list). */
+int  swegl_ctx_set_bin_limit(swegl_ctx *ctx, uint32_t tiles);
 
 /* Per-kernel HIP-event timing on the context stream (0 = off). */
 int  swegl_ctx_set_profiling(swegl_ctx *ctx, int on);

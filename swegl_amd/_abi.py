@@ -75,7 +75,7 @@ class ViewportDesc(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("fragments", C.c_uint64), ("nan_fragments", C.c_uint64),
                 ("drawn_subtriangles", C.c_uint64), ("bin_entries", C.c_uint64),
-                ("texel_oob", C.c_uint64)]
+                ("texel_oob", C.c_uint64), ("big_triangles", C.c_uint64)]
 
 
 class Job(C.Structure):

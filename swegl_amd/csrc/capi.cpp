@@ -107,6 +107,7 @@ struct swegl_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int tw = 32, th = 32;
+    uint32_t big_tiles = 16; // bin limit (swegl_ctx_set_bin_limit)
     DevBuf jobs, tile_count, tile_offset, tile_cursor, bins, ctr;
     unsigned long long *h_stats = nullptr;   // pinned
     StageSlot stage[4];
@@ -213,7 +214,7 @@ extern "C" int swegl_ctx_create(int device, swegl_ctx **out)
     std::unique_ptr<swegl_ctx> c(new swegl_ctx);
     c->device = device;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HIPCHK(hipHostMalloc((void **)&c->h_stats, 32 * sizeof(unsigned long long), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&c->h_stats, SW_CTR_CONTROL * sizeof(uint32_t), hipHostMallocDefault));
     for (StageSlot &s : c->stage)
         HIPCHK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     *out = c.release();
@@ -294,6 +295,16 @@ extern "C" int swegl_ctx_get_tile(swegl_ctx *c, int *tw, int *th)
         *tw = c->tw;
     if (th)
         *th = c->th;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_ctx_set_bin_limit(swegl_ctx *c, uint32_t tiles)
+{
+    if (!c)
+        return fail(SWEGL_EINVAL, "null ctx");
+    if (tiles > 4096)
+        return fail(SWEGL_EINVAL, "bin limit %u above 4096", tiles);
+    c->big_tiles = tiles;
     return SWEGL_OK;
 }
 
@@ -783,7 +794,8 @@ static int run_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_wav
     }
     int rc;
     const uint32_t n_ctr = SW_CTR_BIG0 + nj;
-    const uint64_t cap = std::max<uint64_t>(1, tris_total * SW_BIG_TILES);
+    // every binned triangle contributes at most big_tiles entries
+    const uint64_t cap = std::max<uint64_t>(4096, tris_total * std::max<uint32_t>(1, c->big_tiles));
     if (cap > 0xFFFFFFF0ull)
         return fail(SWEGL_ERANGE, "too many triangles in one batch");
     if ((rc = c->jobs.ensure(nj * sizeof(JobDev))) || (rc = c->tile_count.ensure((size_t)tiles * 4 + 4)) ||
@@ -802,11 +814,10 @@ static int run_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_wav
     B.bins = c->bins.as<uint32_t>();
     B.bin_capacity = (uint32_t)cap;
     B.ctr = c->ctr.as<uint32_t>();
-    // stats words [0,16) are zeroed once per render by the first wave
-    BatchDev Bc = B;
-    if (!first_wave)
-        Bc.ctr = B.ctr + 16;
-    const uint32_t zero_words = first_wave ? n_ctr : n_ctr - 16;
+    B.big_tiles = c->big_tiles;
+    // statistics are zeroed once per render by the first wave; control words
+    // and big-list lengths by every wave
+    const uint32_t zero0 = first_wave ? 0 : SW_CTR_CONTROL;
     hipStream_t st = c->stream;
 #define STAGE(name, call)                                                              \
     do {                                                                               \
@@ -814,7 +825,7 @@ static int run_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_wav
         if (e_ != hipSuccess)                                                          \
             return fail(SWEGL_EIO, "%s launch: %s", name, hipGetErrorString(e_));       \
     } while (0)
-    STAGE("clear", launch_clear(Bc, zero_words, st));
+    STAGE("clear", launch_clear(B, zero0, n_ctr - zero0, st));
     STAGE("vertex", launch_vertex(B, vblk, st));
     STAGE("mark", launch_mark(B, tblk, st));
     STAGE("bin_count", launch_bin_count(B, tblk, st));
@@ -857,15 +868,22 @@ static int render_jobs(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_
             return rc;
     }
     if (stats) {
-        HIPCHK(hipMemcpyAsync(c->h_stats, c->ctr.p, 8 * sizeof(unsigned long long),
+        HIPCHK(hipMemcpyAsync(c->h_stats, c->ctr.p, SW_CTR_CONTROL * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         drain_profile(c);
-        stats->fragments = c->h_stats[SW_ST_FRAGMENTS];
-        stats->nan_fragments = c->h_stats[SW_ST_NAN];
-        stats->drawn_subtriangles = c->h_stats[SW_ST_DRAWN];
-        stats->bin_entries = c->h_stats[SW_ST_BINS];
-        stats->texel_oob = c->h_stats[SW_ST_OOB];
+        const uint32_t *w = (const uint32_t *)c->h_stats;
+        const unsigned long long *sl = (const unsigned long long *)(w + SW_CTR_STATS);
+        uint64_t sum[8] = {0};
+        for (int k = 0; k < SW_STAT_SLOTS; k++)
+            for (int i = 0; i < 8; i++)
+                sum[i] += sl[k * 8 + i];
+        stats->fragments = sum[SW_ST_FRAGMENTS];
+        stats->nan_fragments = sum[SW_ST_NAN];
+        stats->drawn_subtriangles = sum[SW_ST_DRAWN];
+        stats->bin_entries = sum[SW_ST_BINS];
+        stats->texel_oob = sum[SW_ST_OOB];
+        stats->big_triangles = sum[SW_ST_BIG];
     }
     return SWEGL_OK;
 }

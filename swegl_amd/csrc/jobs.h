@@ -15,7 +15,8 @@
 //   ndc                          : float4[n_vertices]  frustum coordinates
 //   yes                          : uint8[n_vertices]   visibility marks
 //   rect                         : uint2[n_tris]       tile rectangle of each triangle
-//   big                          : uint32[n_tris]      triangles covering many tiles
+//   big                          : uint32[n_tris]      triangles covering more than
+//                                                      BatchDev::big_tiles tiles
 #pragma once
 #include <stdint.h>
 #include "core.h"
@@ -42,7 +43,6 @@ struct TexDev {
     int32_t w, h;
 };
 
-#define SW_BIG_TILES 16      // triangles touching more tiles go to the per-job big list
 #define SW_MAX_DOF_RADIUS 16
 
 struct JobDev {
@@ -91,17 +91,24 @@ struct BatchDev {
     uint32_t *tile_cursor;    // [n_tiles]
     uint32_t *bins;           // [capacity]
     uint32_t bin_capacity;
-    uint32_t *ctr;            // counters: [0..15] stats as 8 u64, [16] raster queue,
-                              // [32 + j] big-list length of job j
+    uint32_t big_tiles;       // per-triangle bin limit; larger ones go to the big list
+    uint32_t *ctr;            // counter words, layout below
 };
 
-// Counter words (uint64 pairs in ctr[0..15])
+// Counter block: [0, 1024) statistics as 64 slots x 8 u64 (slot = block
+// index mod 64, so no address takes one atomic per block; the host sums the
+// slots), [1024, 1088) control words, [1088 + j) big-list length of job j.
+#define SW_STAT_SLOTS 64
+#define SW_CTR_STATS 0
+#define SW_CTR_CONTROL 1024
+#define SW_CTR_TOTAL (SW_CTR_CONTROL + 1)
+#define SW_CTR_BIG0 (SW_CTR_CONTROL + 64)
+// statistic indices within a slot
 #define SW_ST_FRAGMENTS 0
 #define SW_ST_NAN 1
 #define SW_ST_DRAWN 2
 #define SW_ST_BINS 3
 #define SW_ST_OOB 4
-#define SW_CTR_QUEUE 16
-#define SW_CTR_BIG0 32
+#define SW_ST_BIG 5           // triangles routed to the big list
 
 }  // namespace sw

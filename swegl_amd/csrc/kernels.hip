@@ -93,14 +93,20 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t *base, uint32_t key, bool 
     return slot;
 }
 
+__device__ __forceinline__ void stat_add(const BatchDev &B, int idx, unsigned long long v)
+{
+    unsigned long long *st = (unsigned long long *)(B.ctr + SW_CTR_STATS);   // 8-byte aligned
+    atomicAdd(&st[(blockIdx.x & (SW_STAT_SLOTS - 1)) * 8 + idx], v);
+}
+
 // ------------------------------------------------------------- k_clear
-__global__ void k_clear(BatchDev B, uint32_t n_words)
+__global__ void k_clear(BatchDev B, uint32_t w0, uint32_t n_words)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < B.n_tiles)
         B.tile_count[i] = 0;
     if (i < n_words)
-        B.ctr[i] = 0;
+        B.ctr[w0 + i] = 0;
 }
 
 // ------------------------------------------------------------ k_vertex
@@ -261,76 +267,128 @@ __device__ bool sub_extent(const Setup &S, const ViewXf &V, Extent &E)
 #define RECT_EMPTY 0xFFFFFFFFu
 
 // ---------------------------------------------------------- k_bin_count
+// Tile rectangle of every triangle, conservative but never too small: rows
+// are exact (ceil of vertex y, clipped to the viewport), columns are the
+// vertex x range widened by the DDA's worst-case accumulated rounding.
+// Near-clipped or non-finite triangles take the exact fill_triangle path.
+// Each block of 256 consecutive triangles publishes the union rectangle and
+// counts one bin entry per tile it covers.
+__device__ bool quick_extent(const JobDev &J, uint32_t t, Extent &E, uint32_t &drawn, bool &exact)
+{
+    exact = false;
+    uint32_t i0 = J.tri[3 * t], i1 = J.tri[3 * t + 1], i2 = J.tri[3 * t + 2];
+    if (!(J.yes[i0] && J.yes[i1] && J.yes[i2]))
+        return false;
+    f3 a = frustum_to_screen(J.V, ld3(J.ndc, i0));
+    f3 b = frustum_to_screen(J.V, ld3(J.ndc, i1));
+    f3 c = frustum_to_screen(J.V, ld3(J.ndc, i2));
+    const float big = 1073741824.0f;
+    bool finite = fabsf(a.x) < big && fabsf(b.x) < big && fabsf(c.x) < big && fabsf(a.y) < big &&
+                  fabsf(b.y) < big && fabsf(c.y) < big;
+    if (!finite || !(fminf(a.z, fminf(b.z, c.z)) >= SW_NEAR) || a.z != a.z || b.z != b.z || c.z != c.z) {
+        exact = true;
+        return false;
+    }
+    const float ymin = fminf(a.y, fminf(b.y, c.y)), ymax = fmaxf(a.y, fmaxf(b.y, c.y));
+    const int iy0 = cvtt(ceilf(ymin)), iy2 = cvtt(ceilf(ymax));
+    if (iy0 == iy2)
+        return false;
+    drawn = 1;
+    const int y0 = max(iy0, J.V.vy), y1 = min(iy2, J.V.vy + J.V.vh);
+    if (y0 >= y1)
+        return false;
+    E.y0 = y0;
+    E.y1 = y1;
+    const float xmin = fminf(a.x, fminf(b.x, c.x)), xmax = fmaxf(a.x, fmaxf(b.x, c.x));
+    const float X = fmaxf(fmaxf(fabsf(xmin), fabsf(xmax)), 1.0f);
+    const float slack = 2.0f + (float)(iy2 - iy0 + 8) * X * (1.0f / 1048576.0f);
+    const float fx0 = floorf(xmin - slack), fx1 = ceilf(xmax + slack);
+    E.x0 = fx0 < (float)J.V.vx ? J.V.vx : (int)fx0;
+    E.x1 = fx1 > (float)(J.V.vx + J.V.vw - 1) ? J.V.vx + J.V.vw - 1 : (int)fx1;
+    return E.x0 <= E.x1;
+}
+
 __global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
 {
     const int jidx = find_job<JT>(B, blockIdx.x);
     const JobDev &J = B.jobs[jidx];
-    uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
-    uint32_t drawn = 0, entries = 0;
-    int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0, ty1 = 0;
+    const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
+    uint32_t drawn = 0, big = 0;
+    int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
     if (t < J.n_tris) {
-        SubTri st[2];
-        int ns = tri_subs(J, t, st);
-        int ya = INT32_MAX, yb = INT32_MIN, xa = INT32_MAX, xb = INT32_MIN;
+        Extent E;
+        bool exact;
+        bool have = quick_extent(J, t, E, drawn, exact);
+        if (exact) {
+            SubTri st[2];
+            int ns = tri_subs(J, t, st);
+            int ya = INT32_MAX, yb = INT32_MIN, xa = INT32_MAX, xb = INT32_MIN;
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-            Setup S;
-            if (s >= ns || !setup_subtri(st[s], J.V, S))
-                continue;
-            drawn++;
-            Extent E;
-            if (!sub_extent(S, J.V, E))
-                continue;
-            ya = min(ya, E.y0);
-            yb = max(yb, E.y1);
-            xa = min(xa, E.x0);
-            xb = max(xb, E.x1);
+            for (int s = 0; s < 2; s++) {
+                Setup S;
+                if (s >= ns || !setup_subtri(st[s], J.V, S))
+                    continue;
+                drawn++;
+                Extent e;
+                if (!sub_extent(S, J.V, e))
+                    continue;
+                ya = min(ya, e.y0);
+                yb = max(yb, e.y1);
+                xa = min(xa, e.x0);
+                xb = max(xb, e.x1);
+            }
+            have = ya < yb && xa <= xb;
+            E.y0 = ya;
+            E.y1 = yb;
+            E.x0 = xa;
+            E.x1 = xb;
         }
         uint2 r = make_uint2(RECT_EMPTY, 0);
-        if (ya < yb && xa <= xb) {
-            tx0 = (xa - J.V.vx) / J.tw;
-            tx1 = (xb - J.V.vx) / J.tw;
-            ty0 = (ya - J.V.vy) / J.th;
-            ty1 = (yb - 1 - J.V.vy) / J.th;
+        if (have) {
+            tx0 = (E.x0 - J.V.vx) / J.tw;
+            tx1 = (E.x1 - J.V.vx) / J.tw;
+            ty0 = (E.y0 - J.V.vy) / J.th;
+            const int ty1 = (E.y1 - 1 - J.V.vy) / J.th;
             r = make_uint2((uint32_t)tx0 | ((uint32_t)ty0 << 16), (uint32_t)tx1 | ((uint32_t)ty1 << 16));
             ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-            if (ntiles > SW_BIG_TILES) {
-                uint32_t pos = atomicAdd(&B.ctr[SW_CTR_BIG0 + jidx], 1u);
-                J.big[pos] = t;
+            if (ntiles > (int)B.big_tiles) {
+                J.big[atomicAdd(&B.ctr[SW_CTR_BIG0 + jidx], 1u)] = t;
                 ntiles = 0;
+                big = 1;
             }
-            entries = ntiles;
         }
         J.rect[t] = r;
     }
     // one aggregated atomic per distinct tile per wave and round
+    uint32_t entries = 0;
     for (int k = 0; __any(k < ntiles); k++) {
-        bool act = k < ntiles;
+        const bool act = k < ntiles;
         uint32_t key = 0;
         if (act) {
-            int w = tx1 - tx0 + 1;
+            const int w = tx1 - tx0 + 1;
             key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
+            entries++;
         }
         wave_add(B.tile_count, key, act);
     }
-    // block totals for the frame statistics
-    __shared__ uint32_t s_drawn, s_entries;
-    if (threadIdx.x == 0) {
-        s_drawn = 0;
-        s_entries = 0;
-    }
+    __shared__ uint32_t s_acc[3];
+    if (threadIdx.x < 3)
+        s_acc[threadIdx.x] = 0;
     __syncthreads();
     if (drawn)
-        atomicAdd(&s_drawn, drawn);
+        atomicAdd(&s_acc[0], drawn);
     if (entries)
-        atomicAdd(&s_entries, entries);
+        atomicAdd(&s_acc[1], entries);
+    if (big)
+        atomicAdd(&s_acc[2], big);
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long *st = (unsigned long long *)B.ctr;
-        if (s_drawn)
-            atomicAdd(&st[SW_ST_DRAWN], (unsigned long long)s_drawn);
-        if (s_entries)
-            atomicAdd(&st[SW_ST_BINS], (unsigned long long)s_entries);
+        if (s_acc[0])
+            stat_add(B, SW_ST_DRAWN, s_acc[0]);
+        if (s_acc[1])
+            stat_add(B, SW_ST_BINS, s_acc[1]);
+        if (s_acc[2])
+            stat_add(B, SW_ST_BIG, s_acc[2]);
     }
 }
 
@@ -361,36 +419,38 @@ __global__ __launch_bounds__(1024) void k_scan(BatchDev B)
         B.tile_cursor[i] = run;
         run += B.tile_count[i];
     }
-    if (threadIdx.x == 1023)
+    if (threadIdx.x == 1023) {
         B.tile_offset[n] = part[1023];
+        B.ctr[SW_CTR_TOTAL] = part[1023];   // total bin entries
+    }
 }
 
 // ----------------------------------------------------------- k_bin_fill
 __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B)
 {
     const JobDev &J = B.jobs[find_job<JT>(B, blockIdx.x)];
-    uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
+    const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
     int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
     if (t < J.n_tris) {
-        uint2 r = J.rect[t];
+        const uint2 r = J.rect[t];
         if (r.x != RECT_EMPTY) {
             tx0 = r.x & 0xFFFF;
             ty0 = r.x >> 16;
             tx1 = r.y & 0xFFFF;
-            int ty1 = r.y >> 16;
+            const int ty1 = r.y >> 16;
             ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-            if (ntiles > SW_BIG_TILES)
+            if (ntiles > (int)B.big_tiles)
                 ntiles = 0;
         }
     }
     for (int k = 0; __any(k < ntiles); k++) {
-        bool act = k < ntiles;
+        const bool act = k < ntiles;
         uint32_t key = 0;
         if (act) {
-            int w = tx1 - tx0 + 1;
+            const int w = tx1 - tx0 + 1;
             key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
         }
-        uint32_t pos = wave_add(B.tile_cursor, key, act);
+        const uint32_t pos = wave_add(B.tile_cursor, key, act);
         if (act && pos < B.bin_capacity)
             B.bins[pos] = t;
     }
@@ -535,7 +595,7 @@ template <int TW, int TH>
 __global__ __launch_bounds__(256) void k_raster(BatchDev B)
 {
     __shared__ unsigned long long keys[TW * TH];
-    __shared__ uint32_t s_cnt[3];
+    __shared__ uint32_t s_cnt[4];
     const uint32_t g = blockIdx.x;
     const int jidx = find_job<JTILE>(B, g);
     const JobDev &J = B.jobs[jidx];
@@ -549,11 +609,13 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
 
     for (int i = threadIdx.x; i < TW * TH; i += 256)
         keys[i] = SW_KEY_EMPTY;
-    if (threadIdx.x < 3)
+    if (threadIdx.x < 4)
         s_cnt[threadIdx.x] = 0;
     __syncthreads();
 
     uint32_t frags = 0, nans = 0, oob = 0;
+    // Replay the tile's binned triangles, then the job's big triangles whose
+    // rectangle touches this tile, one per lane.
     const uint32_t nb = B.tile_count[g], off = B.tile_offset[g];
     const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
     for (uint32_t e = threadIdx.x; e < nb + nbig; e += 256) {
@@ -562,9 +624,9 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
             t = B.bins[off + e];
         } else {
             t = J.big[e - nb];
-            uint2 r = J.rect[t];
-            int rx0 = r.x & 0xFFFF, ry0 = r.x >> 16, rx1 = r.y & 0xFFFF, ry1 = r.y >> 16;
-            if (tx < rx0 || tx > rx1 || ty < ry0 || ty > ry1)
+            const uint2 r = J.rect[t];
+            if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
+                ty > (int)(r.y >> 16))
                 continue;
         }
         raster_depth(J, t, tr, keys, TW, frags, nans);
@@ -600,13 +662,13 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
         atomicAdd(&s_cnt[2], oob);
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long *stc = (unsigned long long *)B.ctr;
         if (s_cnt[0])
-            atomicAdd(&stc[SW_ST_FRAGMENTS], (unsigned long long)s_cnt[0]);
+            stat_add(B, SW_ST_FRAGMENTS, s_cnt[0]);
         if (s_cnt[1])
-            atomicAdd(&stc[SW_ST_NAN], (unsigned long long)s_cnt[1]);
+            stat_add(B, SW_ST_NAN, s_cnt[1]);
         if (s_cnt[2])
-            atomicAdd(&stc[SW_ST_OOB], (unsigned long long)s_cnt[2]);
+            stat_add(B, SW_ST_OOB, s_cnt[2]);
+
     }
 }
 
@@ -716,10 +778,10 @@ __global__ void k_fill32(uint32_t *p, uint64_t n, uint32_t w)
             return e_;                                                         \
     } while (0)
 
-hipError_t launch_clear(const BatchDev &B, uint32_t n_words, hipStream_t s)
+hipError_t launch_clear(const BatchDev &B, uint32_t w0, uint32_t n_words, hipStream_t s)
 {
     uint32_t n = B.n_tiles > n_words ? B.n_tiles : n_words;
-    LAUNCH_CHECK(k_clear, dim3((n + 255) / 256), dim3(256), 0, s, B, n_words);
+    LAUNCH_CHECK(k_clear, dim3((n + 255) / 256), dim3(256), 0, s, B, w0, n_words);
     return hipSuccess;
 }
 hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, hipStream_t s)

@@ -38,6 +38,7 @@ def lib():
             "swegl_ctx_set_tile": (C.c_int, [vp, C.c_int, C.c_int]),
             "swegl_ctx_get_tile": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "swegl_ctx_set_profiling": (C.c_int, [vp, C.c_int]),
+            "swegl_ctx_set_bin_limit": (C.c_int, [vp, C.c_uint32]),
             "swegl_ctx_kernel_times": (C.c_int, [vp, C.c_char_p, C.c_int, C.POINTER(C.c_double),
                                                  C.POINTER(C.c_uint64), C.c_int,
                                                  C.POINTER(C.c_int)]),
@@ -123,6 +124,9 @@ class Context:
         w, h = C.c_int(), C.c_int()
         check(lib().swegl_ctx_get_tile(self.h, C.byref(w), C.byref(h)), "swegl_ctx_get_tile")
         return w.value, h.value
+
+    def set_bin_limit(self, tiles):
+        check(lib().swegl_ctx_set_bin_limit(self.h, int(tiles)), "swegl_ctx_set_bin_limit")
 
     def set_profiling(self, on):
         check(lib().swegl_ctx_set_profiling(self.h, int(on)), "swegl_ctx_set_profiling")

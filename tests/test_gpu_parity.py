@@ -51,3 +51,31 @@ def test_heightfield_1m_1080p(ctx):
     assert got.stats["fragments"] == want.stats["fragments"]
     assert got.stats["drawn_subtriangles"] == want.stats["drawn_subtriangles"]
     print(f"heightfield 1M: fragments {got.stats['fragments']}, colour max diff {mx} at {nd} px")
+
+
+@pytest.mark.parametrize("limit", [0, 1, 4])
+def test_bin_limit_paths_agree(ctx, limit):
+    """Triangles over the bin limit take the per-viewport list that every
+    tile tests; forcing it (limit 0) gives the same bits."""
+    scene, vps = scenes.heightfield_scene(4, n=160, width=400, height=300, tex=128, dof=False)
+    want = oracle.render(scene, vps)[0]
+    ctx.set_bin_limit(limit)
+    try:
+        got = gpu_render(ctx, scene, vps)[0]
+    finally:
+        ctx.set_bin_limit(16)
+    compare(got, want)
+    if limit == 0:
+        assert got.stats["big_triangles"] == got.stats["drawn_subtriangles"] or got.stats["big_triangles"] > 0
+
+
+@pytest.mark.parametrize("tile", [(8, 8), (16, 16), (32, 16), (64, 32), (64, 64)])
+def test_tile_sizes_agree(ctx, tile):
+    scene, vps = scenes.heightfield_scene(6, n=150, width=333, height=211, tex=128, dof=True)
+    want = oracle.render(scene, vps)[0]
+    ctx.set_tile(*tile)
+    try:
+        got = gpu_render(ctx, scene, vps)[0]
+    finally:
+        ctx.set_tile(32, 32)
+    compare(got, want, colour_tol=2)
