@@ -236,6 +236,10 @@ typedef struct swegl_job {
 } swegl_job;
 int  swegl_render_batch(swegl_ctx *ctx, const swegl_job *jobs, uint32_t n_jobs,
                         swegl_frame_stats *stats);
+/* The same batch as a replayed hipGraph (captured on the first call with a
+ * given launch shape: job count, scene sizes, viewport sizes, post shaders). */
+int  swegl_render_batch_graph(swegl_ctx *ctx, const swegl_job *jobs, uint32_t n_jobs,
+                              swegl_frame_stats *stats);
 
 /* ---- host math helpers (same float semantics as the reference) --------- */
 /* matrix44_t::rotate_x/y/z (src/projection/matrix44.cpp:12-67), translate

@@ -103,12 +103,48 @@ struct StageSlot {
     bool used = false;
 };
 
+// Device buffers one launch sequence uses.  The eager path keeps one set per
+// context; every captured graph owns a set, since their addresses are baked
+// into the graph's kernel arguments.
+struct Resources {
+    DevBuf jobs, tile_count, tile_offset, tile_cursor, bins, ctr;
+};
+
+// A captured launch sequence.  Only the resource addresses and grid sizes
+// are baked in; every pointer the kernels follow lives in the job table,
+// which is rewritten into the graph's pinned buffer and copied by the
+// graph's first node on every replay.
+struct GraphEntry {
+    Resources res;
+    void *pinned[2] = {nullptr, nullptr};
+    size_t job_bytes = 0;
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    int next = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t copy_node = nullptr;
+    ~GraphEntry()
+    {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        for (int i = 0; i < 2; i++) {
+            if (pinned[i])
+                (void)hipHostFree(pinned[i]);
+            if (done[i])
+                (void)hipEventDestroy(done[i]);
+        }
+    }
+};
+
 struct swegl_ctx {
     int device = 0;
+    int n_cus = 256;
     hipStream_t stream = nullptr;
     int tw = 32, th = 32;
     uint32_t big_tiles = 16; // bin limit (swegl_ctx_set_bin_limit)
-    DevBuf jobs, tile_count, tile_offset, tile_cursor, bins, ctr;
     unsigned long long *h_stats = nullptr;   // pinned
     StageSlot stage[4];
     int stage_next = 0;
@@ -116,6 +152,8 @@ struct swegl_ctx {
     std::vector<ProfRec> pending;
     std::map<std::string, std::pair<double, uint64_t>> prof_acc;
     std::vector<hipEvent_t> event_pool;
+    Resources res;
+    std::map<std::string, std::unique_ptr<GraphEntry>> graphs;
 };
 
 // --------------------------------------------------------------- helpers
@@ -213,6 +251,7 @@ extern "C" int swegl_ctx_create(int device, swegl_ctx **out)
                     prop.gcnArchName);
     std::unique_ptr<swegl_ctx> c(new swegl_ctx);
     c->device = device;
+    c->n_cus = prop.multiProcessorCount;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc((void **)&c->h_stats, SW_CTR_CONTROL * sizeof(uint32_t), hipHostMallocDefault));
     for (StageSlot &s : c->stage)
@@ -238,12 +277,7 @@ extern "C" int swegl_ctx_destroy(swegl_ctx *c)
     }
     if (c->h_stats)
         (void)hipHostFree(c->h_stats);
-    c->jobs.release();
-    c->tile_count.release();
-    c->tile_offset.release();
-    c->tile_cursor.release();
-    c->bins.release();
-    c->ctr.release();
+    c->graphs.clear();
     (void)hipStreamDestroy(c->stream);
     delete c;
     return SWEGL_OK;
@@ -697,13 +731,21 @@ static bool conflicts(const swegl_viewport_desc &a, const swegl_viewport_desc &b
 
 static void m34_from(const float *m16, m34 &o) { memcpy(o.m, m16, 12 * sizeof(float)); }
 
-// Enqueue one wave of independent jobs.
-static int run_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_wave)
+
+// One wave of independent jobs, prepared (buffers allocated, job table
+// built) before anything is launched.
+struct Wave {
+    std::vector<JobDev> jd;
+    uint32_t vblk = 0, tblk = 0, dblk = 0, tiles = 0, n_ctr = 0;
+    uint64_t cap = 0;
+    size_t job_offset = 0;   // byte offset of this wave's table in Resources::jobs
+};
+
+static int build_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_wave, Wave &W)
 {
     const uint32_t nj = (uint32_t)jobs.size();
-    std::vector<JobDev> jd(nj);
+    W.jd.assign(nj, JobDev{});
     std::map<swegl_scene *, int> scene_uses;
-    uint32_t vblk = 0, tblk = 0, dblk = 0, tiles = 0;
     uint64_t tris_total = 0;
     for (uint32_t j = 0; j < nj; j++) {
         swegl_scene *s = jobs[j].scene;
@@ -720,7 +762,7 @@ static int run_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_wav
         const bool dof = v.post_shader != SWEGL_POST_NULL;
         if (dof && (rc = vs.dof.ensure((size_t)v.w * v.h * 4)))
             return rc;
-        JobDev &J = jd[j];
+        JobDev &J = W.jd[j];
         memset(&J, 0, sizeof J);
         J.px = s->px.as<float>();
         J.py = s->py.as<float>();
@@ -781,63 +823,84 @@ static int run_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_wav
         J.th = c->th;
         J.tiles_x = (v.w + c->tw - 1) / c->tw;
         J.tiles_y = (v.h + c->th - 1) / c->th;
-        J.tile_base = tiles;
-        tiles += (uint32_t)(J.tiles_x * J.tiles_y);
-        J.vblock0 = vblk;
-        vblk += (s->n_vertices + 255) / 256;
-        J.tblock0 = tblk;
-        tblk += (s->n_tris + 255) / 256;
-        J.dblock0 = dblk;
+        J.tile_base = W.tiles;
+        W.tiles += (uint32_t)(J.tiles_x * J.tiles_y);
+        J.vblock0 = W.vblk;
+        W.vblk += (s->n_vertices + 255) / 256;
+        J.tblock0 = W.tblk;
+        W.tblk += (s->n_tris + 255) / 256;
+        J.dblock0 = W.dblk;
         if (dof)
-            dblk += (uint32_t)(((v.w + 15) / 16) * ((v.h + 15) / 16));
+            W.dblk += (uint32_t)(((v.w + 15) / 16) * ((v.h + 15) / 16));
         tris_total += s->n_tris;
     }
-    int rc;
-    const uint32_t n_ctr = SW_CTR_BIG0 + nj;
+    W.n_ctr = SW_CTR_BIG0 + nj;
     // every binned triangle contributes at most big_tiles entries
-    const uint64_t cap = std::max<uint64_t>(4096, tris_total * std::max<uint32_t>(1, c->big_tiles));
-    if (cap > 0xFFFFFFF0ull)
+    W.cap = std::max<uint64_t>(4096, tris_total * std::max<uint32_t>(1, c->big_tiles));
+    if (W.cap > 0xFFFFFFF0ull)
         return fail(SWEGL_ERANGE, "too many triangles in one batch");
-    if ((rc = c->jobs.ensure(nj * sizeof(JobDev))) || (rc = c->tile_count.ensure((size_t)tiles * 4 + 4)) ||
-        (rc = c->tile_offset.ensure((size_t)tiles * 4 + 8)) || (rc = c->tile_cursor.ensure((size_t)tiles * 4 + 4)) ||
-        (rc = c->bins.ensure(cap * 4)) || (rc = c->ctr.ensure((size_t)n_ctr * 4)))
+    return SWEGL_OK;
+}
+
+// Size a resource set for a list of waves (grow-only).
+static int size_resources(Resources &R, std::vector<Wave> &waves)
+{
+    size_t job_bytes = 0;
+    uint32_t tiles = 0, n_ctr = 0;
+    uint64_t cap = 0;
+    for (Wave &W : waves) {
+        W.job_offset = job_bytes;
+        job_bytes += (W.jd.size() * sizeof(JobDev) + 255) & ~(size_t)255;
+        tiles = std::max(tiles, W.tiles);
+        n_ctr = std::max(n_ctr, W.n_ctr);
+        cap = std::max(cap, W.cap);
+    }
+    int rc;
+    if ((rc = R.jobs.ensure(job_bytes)) || (rc = R.tile_count.ensure((size_t)tiles * 4 + 4)) ||
+        (rc = R.tile_offset.ensure((size_t)tiles * 4 + 8)) || (rc = R.tile_cursor.ensure((size_t)tiles * 4 + 4)) ||
+        (rc = R.bins.ensure(cap * 4)) || (rc = R.ctr.ensure((size_t)n_ctr * 4)))
         return rc;
-    if ((rc = upload(c, c->jobs.p, jd.data(), nj * sizeof(JobDev))))
-        return rc;
+    return SWEGL_OK;
+}
+
+// Enqueue the kernels of one wave (no allocation: safe under graph capture).
+static int launch_wave(swegl_ctx *c, Resources &R, const Wave &W, bool first_wave, bool capturing)
+{
     BatchDev B;
-    B.jobs = c->jobs.as<JobDev>();
-    B.n_jobs = nj;
-    B.n_tiles = tiles;
-    B.tile_count = c->tile_count.as<uint32_t>();
-    B.tile_offset = c->tile_offset.as<uint32_t>();
-    B.tile_cursor = c->tile_cursor.as<uint32_t>();
-    B.bins = c->bins.as<uint32_t>();
-    B.bin_capacity = (uint32_t)cap;
-    B.ctr = c->ctr.as<uint32_t>();
+    B.jobs = (const JobDev *)((char *)R.jobs.p + W.job_offset);
+    B.n_jobs = (uint32_t)W.jd.size();
+    B.n_tiles = W.tiles;
+    B.tile_count = R.tile_count.as<uint32_t>();
+    B.tile_offset = R.tile_offset.as<uint32_t>();
+    B.tile_cursor = R.tile_cursor.as<uint32_t>();
+    B.bins = R.bins.as<uint32_t>();
+    B.bin_capacity = (uint32_t)W.cap;
+    B.ctr = R.ctr.as<uint32_t>();
     B.big_tiles = c->big_tiles;
     // statistics are zeroed once per render by the first wave; control words
     // and big-list lengths by every wave
     const uint32_t zero0 = first_wave ? 0 : SW_CTR_CONTROL;
+    const uint32_t groups = (uint32_t)std::max(1, c->n_cus) * 4;
     hipStream_t st = c->stream;
 #define STAGE(name, call)                                                              \
     do {                                                                               \
-        hipError_t e_ = timed(c, name, false, [&] { return call; });                    \
+        hipError_t e_ = timed(c, name, capturing, [&] { return call; });                \
         if (e_ != hipSuccess)                                                          \
             return fail(SWEGL_EIO, "%s launch: %s", name, hipGetErrorString(e_));       \
     } while (0)
-    STAGE("clear", launch_clear(B, zero0, n_ctr - zero0, st));
-    STAGE("vertex", launch_vertex(B, vblk, st));
-    STAGE("mark", launch_mark(B, tblk, st));
-    STAGE("bin_count", launch_bin_count(B, tblk, st));
+    STAGE("clear", launch_clear(B, zero0, W.n_ctr - zero0, st));
+    STAGE("vertex", launch_vertex(B, W.vblk, st));
+    STAGE("mark", launch_mark(B, W.tblk, st));
+    STAGE("bin_count", launch_bin_count(B, W.tblk, st));
     STAGE("scan", launch_scan(B, st));
-    STAGE("bin_fill", launch_bin_fill(B, tblk, st));
-    STAGE("raster", launch_raster(B, c->tw, c->th, st));
-    STAGE("dof", launch_dof(B, dblk, st));
+    STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
+    STAGE("raster", launch_raster(B, c->tw, c->th, groups, st));
+    STAGE("dof", launch_dof(B, W.dblk, st));
 #undef STAGE
     return SWEGL_OK;
 }
 
-static int render_jobs(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
+static int prepare(swegl_ctx *c, const std::vector<JobIn> &all, std::vector<Wave> &waves)
 {
     HIPCHK(hipSetDevice(c->device));
     for (const JobIn &j : all) {
@@ -850,42 +913,142 @@ static int render_jobs(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_
     // waves: a job conflicting with an earlier job of the current wave (same
     // surface region or planes) starts a new wave, which keeps the
     // reference's viewport order for overlapping viewports.
-    std::vector<std::vector<JobIn>> waves(1);
+    std::vector<std::vector<JobIn>> groups(1);
     for (const JobIn &j : all) {
         bool clash = false;
-        for (const JobIn &k : waves.back())
+        for (const JobIn &k : groups.back())
             if (conflicts(*j.vp, *k.vp))
                 clash = true;
         if (clash)
-            waves.emplace_back();
-        waves.back().push_back(j);
+            groups.emplace_back();
+        groups.back().push_back(j);
     }
-    for (size_t w = 0; w < waves.size(); w++) {
-        if (waves[w].empty())
+    waves.clear();
+    for (size_t w = 0; w < groups.size(); w++) {
+        if (groups[w].empty())
             continue;
-        int rc = run_wave(c, waves[w], w == 0);
+        waves.emplace_back();
+        int rc = build_wave(c, groups[w], waves.size() == 1, waves.back());
         if (rc)
             return rc;
     }
-    if (stats) {
-        HIPCHK(hipMemcpyAsync(c->h_stats, c->ctr.p, SW_CTR_CONTROL * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        drain_profile(c);
-        const uint32_t *w = (const uint32_t *)c->h_stats;
-        const unsigned long long *sl = (const unsigned long long *)(w + SW_CTR_STATS);
-        uint64_t sum[8] = {0};
-        for (int k = 0; k < SW_STAT_SLOTS; k++)
-            for (int i = 0; i < 8; i++)
-                sum[i] += sl[k * 8 + i];
-        stats->fragments = sum[SW_ST_FRAGMENTS];
-        stats->nan_fragments = sum[SW_ST_NAN];
-        stats->drawn_subtriangles = sum[SW_ST_DRAWN];
-        stats->bin_entries = sum[SW_ST_BINS];
-        stats->texel_oob = sum[SW_ST_OOB];
-        stats->big_triangles = sum[SW_ST_BIG];
-    }
     return SWEGL_OK;
+}
+
+static int read_stats(swegl_ctx *c, Resources &R, swegl_frame_stats *stats)
+{
+    HIPCHK(hipMemcpyAsync(c->h_stats, R.ctr.p, SW_CTR_CONTROL * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_profile(c);
+    const unsigned long long *sl = (const unsigned long long *)((const uint32_t *)c->h_stats + SW_CTR_STATS);
+    uint64_t sum[8] = {0};
+    for (int k = 0; k < SW_STAT_SLOTS; k++)
+        for (int i = 0; i < 8; i++)
+            sum[i] += sl[k * 8 + i];
+    stats->fragments = sum[SW_ST_FRAGMENTS];
+    stats->nan_fragments = sum[SW_ST_NAN];
+    stats->drawn_subtriangles = sum[SW_ST_DRAWN];
+    stats->bin_entries = sum[SW_ST_BINS];
+    stats->texel_oob = sum[SW_ST_OOB];
+    stats->big_triangles = sum[SW_ST_BIG];
+    return SWEGL_OK;
+}
+
+static int render_jobs(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
+{
+    std::vector<Wave> waves;
+    int rc = prepare(c, all, waves);
+    if (rc)
+        return rc;
+    if ((rc = size_resources(c->res, waves)))
+        return rc;
+    for (size_t w = 0; w < waves.size(); w++) {
+        if ((rc = upload(c, (char *)c->res.jobs.p + waves[w].job_offset, waves[w].jd.data(),
+                         waves[w].jd.size() * sizeof(JobDev))) ||
+            (rc = launch_wave(c, c->res, waves[w], w == 0, false)))
+            return rc;
+    }
+    return stats ? read_stats(c, c->res, stats) : SWEGL_OK;
+}
+
+// ----------------------------------------------------------- graph path
+
+static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
+{
+    std::string k = std::to_string(c->tw) + "x" + std::to_string(c->th) + "/" + std::to_string(c->big_tiles);
+    for (const Wave &W : waves) {
+        k += "|" + std::to_string(W.jd.size()) + ":" + std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
+             "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap);
+    }
+    return k;
+}
+
+static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
+{
+    std::vector<Wave> waves;
+    int rc = prepare(c, all, waves);
+    if (rc)
+        return rc;
+    const std::string key = graph_key(c, waves);
+    auto it = c->graphs.find(key);
+    GraphEntry *G;
+    if (it == c->graphs.end()) {
+        std::unique_ptr<GraphEntry> g(new GraphEntry);
+        if ((rc = size_resources(g->res, waves)))
+            return rc;
+        for (const Wave &W : waves)
+            g->job_bytes = std::max(g->job_bytes, W.job_offset + W.jd.size() * sizeof(JobDev));
+        for (int i = 0; i < 2; i++) {
+            HIPCHK(hipHostMalloc(&g->pinned[i], g->job_bytes, hipHostMallocDefault));
+            HIPCHK(hipEventCreateWithFlags(&g->done[i], hipEventDisableTiming));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipMemcpyAsync(g->res.jobs.p, g->pinned[0], g->job_bytes, hipMemcpyHostToDevice,
+                                      c->stream);
+        int lrc = e == hipSuccess ? SWEGL_OK : fail(SWEGL_EIO, "capture copy: %s", hipGetErrorString(e));
+        for (size_t w = 0; lrc == SWEGL_OK && w < waves.size(); w++)
+            lrc = launch_wave(c, g->res, waves[w], w == 0, true);
+        hipGraph_t graph = nullptr;
+        hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        if (lrc)
+            return lrc;
+        if (ee != hipSuccess)
+            return fail(SWEGL_EIO, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+        g->graph = graph;
+        size_t nn = 0;
+        HIPCHK(hipGraphGetNodes(graph, nullptr, &nn));
+        std::vector<hipGraphNode_t> nodes(nn);
+        HIPCHK(hipGraphGetNodes(graph, nodes.data(), &nn));
+        for (hipGraphNode_t n : nodes) {
+            hipGraphNodeType t;
+            HIPCHK(hipGraphNodeGetType(n, &t));
+            if (t == hipGraphNodeTypeMemcpy)
+                g->copy_node = n;
+        }
+        if (!g->copy_node)
+            return fail(SWEGL_EIO, "captured graph has no job-table copy");
+        HIPCHK(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
+        G = g.get();
+        c->graphs[key] = std::move(g);
+    } else {
+        G = it->second.get();
+    }
+    // write this frame's job tables into the next pinned buffer, once the
+    // replay that last read it has finished
+    const int slot = G->next;
+    G->next ^= 1;
+    if (G->used[slot])
+        HIPCHK(hipEventSynchronize(G->done[slot]));
+    for (const Wave &W : waves)
+        memcpy((char *)G->pinned[slot] + W.job_offset, W.jd.data(), W.jd.size() * sizeof(JobDev));
+    HIPCHK(hipGraphExecMemcpyNodeSetParams1D(G->exec, G->copy_node, G->res.jobs.p, G->pinned[slot],
+                                             G->job_bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipGraphLaunch(G->exec, c->stream));
+    HIPCHK(hipEventRecord(G->done[slot], c->stream));
+    G->used[slot] = true;
+    return stats ? read_stats(c, G->res, stats) : SWEGL_OK;
 }
 
 extern "C" int swegl_render(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vps, uint32_t n,
@@ -902,7 +1065,23 @@ extern "C" int swegl_render(swegl_ctx *c, swegl_scene *s, const swegl_viewport_d
 extern "C" int swegl_render_graph(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vps,
                                   uint32_t n, swegl_frame_stats *stats)
 {
-    return swegl_render(c, s, vps, n, stats);
+    if (!c || !s || (n && !vps))
+        return fail(SWEGL_EINVAL, "null argument");
+    std::vector<JobIn> jobs;
+    for (uint32_t i = 0; i < n; i++)
+        jobs.push_back({s, &vps[i]});
+    return render_jobs_graph(c, jobs, stats);
+}
+
+static int collect(const swegl_job *jobs, uint32_t n, std::vector<JobIn> &all)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        if (jobs[i].n_viewports && !jobs[i].viewports)
+            return fail(SWEGL_EINVAL, "job %u has no viewports", i);
+        for (uint32_t k = 0; k < jobs[i].n_viewports; k++)
+            all.push_back({jobs[i].scene, &jobs[i].viewports[k]});
+    }
+    return SWEGL_OK;
 }
 
 extern "C" int swegl_render_batch(swegl_ctx *c, const swegl_job *jobs, uint32_t n, swegl_frame_stats *stats)
@@ -910,10 +1089,18 @@ extern "C" int swegl_render_batch(swegl_ctx *c, const swegl_job *jobs, uint32_t 
     if (!c || (n && !jobs))
         return fail(SWEGL_EINVAL, "null argument");
     std::vector<JobIn> all;
-    for (uint32_t i = 0; i < n; i++)
-        for (uint32_t k = 0; k < jobs[i].n_viewports; k++)
-            all.push_back({jobs[i].scene, &jobs[i].viewports[k]});
-    return render_jobs(c, all, stats);
+    int rc = collect(jobs, n, all);
+    return rc ? rc : render_jobs(c, all, stats);
+}
+
+extern "C" int swegl_render_batch_graph(swegl_ctx *c, const swegl_job *jobs, uint32_t n,
+                                        swegl_frame_stats *stats)
+{
+    if (!c || (n && !jobs))
+        return fail(SWEGL_EINVAL, "null argument");
+    std::vector<JobIn> all;
+    int rc = collect(jobs, n, all);
+    return rc ? rc : render_jobs_graph(c, all, stats);
 }
 
 // ----------------------------------------------------- host math exports

@@ -67,30 +67,42 @@ __device__ __forceinline__ f3 ld3(const float4 *a, uint32_t i)
     return {v.x, v.y, v.z};
 }
 
-// Wave-aggregated atomicAdd: lanes holding the same key share one atomic
-// (ballot / readlane per distinct key).  Returns this lane's slot
-// (old value + rank among same-key lanes); inactive lanes get 0.
+// Wave-aggregated atomicAdd.  Lanes holding the same key are grouped with
+// ballots (ALU only); then every group leader issues its atomic in the same
+// instruction, so the wave waits for one round trip instead of one per key.
+// Returns this lane's slot (old value + rank among same-key lanes) when
+// `want_slot`; inactive lanes get 0.
+template <bool want_slot>
 __device__ __forceinline__ uint32_t wave_add(uint32_t *base, uint32_t key, bool active)
 {
     const int lane = __lane_id();
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t pending = __ballot(active);
-    uint32_t slot = 0;
+    int my_leader = lane;
+    uint32_t my_rank = 0, count = 0;
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lkey = __shfl(key, leader);
         const uint64_t same = __ballot(active && key == lkey);
-        uint32_t old = 0;
-        if (lane == leader)
-            old = atomicAdd(&base[lkey], (uint32_t)__popcll(same));
-        old = __shfl(old, leader);
         if (active && key == lkey) {
-            slot = old + (uint32_t)__popcll(same & lt);
-            active = false;
+            my_leader = leader;
+            my_rank = (uint32_t)__popcll(same & below);
         }
+        if (lane == leader)
+            count = (uint32_t)__popcll(same);
         pending &= ~same;
     }
-    return slot;
+    const bool is_leader = active && my_leader == lane;
+    if (!want_slot) {
+        if (is_leader)
+            atomicAdd(&base[key], count);
+        return 0;
+    }
+    uint32_t old = 0;
+    if (is_leader)
+        old = atomicAdd(&base[key], count);
+    old = __shfl(old, my_leader);
+    return active ? old + my_rank : 0u;
 }
 
 __device__ __forceinline__ void stat_add(const BatchDev &B, int idx, unsigned long long v)
@@ -369,7 +381,7 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
             key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
             entries++;
         }
-        wave_add(B.tile_count, key, act);
+        wave_add<false>(B.tile_count, key, act);
     }
     __shared__ uint32_t s_acc[3];
     if (threadIdx.x < 3)
@@ -450,7 +462,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B)
             const int w = tx1 - tx0 + 1;
             key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
         }
-        const uint32_t pos = wave_add(B.tile_cursor, key, act);
+        const uint32_t pos = wave_add<true>(B.tile_cursor, key, act);
         if (act && pos < B.bin_capacity)
             B.bins[pos] = t;
     }
@@ -592,11 +604,8 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
 }
 
 template <int TW, int TH>
-__global__ __launch_bounds__(256) void k_raster(BatchDev B)
+__device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *keys, uint32_t *s_cnt)
 {
-    __shared__ unsigned long long keys[TW * TH];
-    __shared__ uint32_t s_cnt[4];
-    const uint32_t g = blockIdx.x;
     const int jidx = find_job<JTILE>(B, g);
     const JobDev &J = B.jobs[jidx];
     const int lt = (int)(g - J.tile_base);
@@ -609,8 +618,6 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
 
     for (int i = threadIdx.x; i < TW * TH; i += 256)
         keys[i] = SW_KEY_EMPTY;
-    if (threadIdx.x < 4)
-        s_cnt[threadIdx.x] = 0;
     __syncthreads();
 
     uint32_t frags = 0, nans = 0, oob = 0;
@@ -653,13 +660,34 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
         if (J.tid)
             J.tid[zo] = seq;
     }
-
     if (frags)
         atomicAdd(&s_cnt[0], frags);
     if (nans)
         atomicAdd(&s_cnt[1], nans);
     if (oob)
         atomicAdd(&s_cnt[2], oob);
+}
+
+// Persistent: each workgroup takes tiles from the batch's work queue until
+// the queue passes the last tile, so tiles of many jobs share the machine.
+template <int TW, int TH>
+__global__ __launch_bounds__(256) void k_raster(BatchDev B)
+{
+    __shared__ unsigned long long keys[TW * TH];
+    __shared__ uint32_t s_cnt[3];
+    __shared__ uint32_t s_tile;
+    if (threadIdx.x < 3)
+        s_cnt[threadIdx.x] = 0;
+    // first tile by block index, the rest from the queue
+    uint32_t g = blockIdx.x;
+    while (g < B.n_tiles) {
+        raster_tile<TW, TH>(B, g, keys, s_cnt);
+        __syncthreads();   // tile done with keys before the next one clears them
+        if (threadIdx.x == 0)
+            s_tile = gridDim.x + atomicAdd(&B.ctr[SW_CTR_CONTROL], 1u);
+        __syncthreads();
+        g = s_tile;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         if (s_cnt[0])
@@ -668,7 +696,6 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
             stat_add(B, SW_ST_NAN, s_cnt[1]);
         if (s_cnt[2])
             stat_add(B, SW_ST_OOB, s_cnt[2]);
-
     }
 }
 
@@ -813,11 +840,12 @@ hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s)
         LAUNCH_CHECK(k_bin_fill, dim3(blocks), dim3(256), 0, s, B);
     return hipSuccess;
 }
-hipError_t launch_raster(const BatchDev &B, int tw, int th, hipStream_t s)
+hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups, hipStream_t s)
 {
     if (!B.n_tiles)
         return hipSuccess;
-    dim3 g(B.n_tiles), b(256);
+    const uint32_t groups = B.n_tiles < max_groups ? B.n_tiles : max_groups;
+    dim3 g(groups), b(256);
     if (tw == 8 && th == 8)
         LAUNCH_CHECK((k_raster<8, 8>), g, b, 0, s, B);
     else if (tw == 16 && th == 16)
@@ -834,6 +862,7 @@ hipError_t launch_raster(const BatchDev &B, int tw, int th, hipStream_t s)
         return hipErrorInvalidValue;
     return hipSuccess;
 }
+
 hipError_t launch_dof(const BatchDev &B, uint32_t blocks, hipStream_t s)
 {
     if (blocks)

@@ -57,6 +57,8 @@ def lib():
                                              C.POINTER(A.FrameStats)]),
             "swegl_render_batch": (C.c_int, [vp, C.POINTER(A.Job), C.c_uint32,
                                              C.POINTER(A.FrameStats)]),
+            "swegl_render_batch_graph": (C.c_int, [vp, C.POINTER(A.Job), C.c_uint32,
+                                                   C.POINTER(A.FrameStats)]),
             "swegl_matrix_rotate_x": (None, [A.FP, C.c_float]),
             "swegl_matrix_rotate_y": (None, [A.FP, C.c_float]),
             "swegl_matrix_rotate_z": (None, [A.FP, C.c_float]),

@@ -61,3 +61,53 @@ def compare(got, want, colour_tol=1, check_tid=True):
     ndiff = int((d.reshape(got.pixels.shape + (4,)).max(-1) > 0).sum())
     assert mx <= colour_tol, f"colour differs by up to {mx} LSB at {ndiff} pixels"
     return mx, ndiff
+
+
+class Batch:
+    """Device state for a list of (scene, viewports) jobs."""
+
+    def __init__(self, ctx, pairs, with_tid=True):
+        self.ctx = ctx
+        self.pairs = pairs
+        self.dscenes = [DeviceScene(ctx, SceneDescHolder(s)) for s, _ in pairs]
+        self.targets = []
+        self.descs = []
+        for s, vps in pairs:
+            arr = (A.ViewportDesc * len(vps))()
+            tl = []
+            for i, vp in enumerate(vps):
+                col = Target(ctx, A.TARGET_COLOR, vp.m_screen.w, vp.m_screen.h)
+                col.write(vp.m_screen.pixels)
+                z = Target(ctx, A.TARGET_FLOAT, vp.m_w, vp.m_h)
+                tid = Target(ctx, A.TARGET_UINT, vp.m_w, vp.m_h) if with_tid else None
+                arr[i] = vp.desc(col, z, tid)
+                tl.append((col, z, tid))
+            self.descs.append(arr)
+            self.targets.append(tl)
+        self.jobs = (A.Job * len(pairs))()
+        for k, (s, vps) in enumerate(pairs):
+            self.jobs[k] = A.Job(self.dscenes[k].h, self.descs[k], len(vps))
+
+    def refresh_cameras(self):
+        for k, (s, vps) in enumerate(self.pairs):
+            for i, vp in enumerate(vps):
+                col, z, tid = self.targets[k][i]
+                self.descs[k][i] = vp.desc(col, z, tid)
+
+    def render(self, graph=False, stats=True):
+        st = (A.FrameStats * 1)()
+        fn = lib().swegl_render_batch_graph if graph else lib().swegl_render_batch
+        check(fn(self.ctx.h, self.jobs, len(self.pairs), st if stats else None), "render_batch")
+        if not stats:
+            self.ctx.sync()
+        return {k: getattr(st[0], k) for k, _ in A.FrameStats._fields_}
+
+    def frames(self):
+        return [[GpuFrame(c.read(), z.read(), t.read() if t is not None else None, None)
+                 for (c, z, t) in tl] for tl in self.targets]
+
+
+def same_frame(a, b):
+    return (np.array_equal(a.pixels, b.pixels) and np.array_equal(a.zbuffer.view(np.uint32),
+                                                                 b.zbuffer.view(np.uint32))
+            and (a.tid is None or b.tid is None or np.array_equal(a.tid, b.tid)))
