@@ -62,6 +62,144 @@ def cpu_baseline(scene, vps, budget_s):
             "sample": f"{n} frames of the same workload, single thread, {el:.1f} s ({cpu})"}
 
 
+def bench_batch(ctx, A, check, L, count, seed, iters):
+    """Workload 'batch': `count` height fields (T log-uniform in [1e3, 2e5],
+    square framebuffers of 256..1024, DoF on half).  Plain = one graph
+    replay per scene; batched = one graph replay for the whole set."""
+    import numpy as np
+    from swegl_amd import scenes
+    from swegl_amd.device import DeviceScene, Target
+    from swegl_amd.host import SceneDescHolder
+
+    t0 = time.perf_counter()
+    pairs = scenes.batch_scenes(seed, count=count)
+    gen_s = time.perf_counter() - t0
+    ds = [DeviceScene(ctx, SceneDescHolder(s)) for s, _ in pairs]
+    keep, descs = [], []
+    for s, vps in pairs:
+        vp = vps[0]
+        col = Target(ctx, A.TARGET_COLOR, vp.m_screen.w, vp.m_screen.h)
+        z = Target(ctx, A.TARGET_FLOAT, vp.m_w, vp.m_h)
+        keep += [col, z]
+        arr = (A.ViewportDesc * 1)()
+        arr[0] = vp.desc(col, z)
+        descs.append(arr)
+    jobs = (A.Job * count)()
+    for k in range(count):
+        jobs[k] = A.Job(ds[k].h, descs[k], 1)
+
+    def plain():
+        for k in range(count):
+            check(L.swegl_render_graph(ctx.h, ds[k].h, descs[k], 1, None), "render_graph")
+
+    def batched():
+        check(L.swegl_render_batch_graph(ctx.h, jobs, count, None), "render_batch_graph")
+
+    st = (A.FrameStats * 1)()
+    check(L.swegl_render_batch(ctx.h, jobs, count, st), "render_batch")
+    frags = st[0].fragments
+    out = {"scenes": count, "triangles": int(sum(s.n_triangles() for s, _ in pairs)),
+           "pixels": int(sum(v[0].m_w * v[0].m_h for _, v in pairs)), "fragments": int(frags),
+           "generate_s": round(gen_s, 1)}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        fn()
+        fn()
+        ctx.sync()
+        a = time.perf_counter()
+        for _ in range(iters):
+            fn()
+        ctx.sync()
+        el = time.perf_counter() - a
+        fps = count * iters / el
+        out[name] = {"frames_per_s": fps, "gfragments_per_s": frags * iters / el / 1e9,
+                     "ms_per_batch": el * 1e3 / iters}
+    out["batched_over_plain"] = out["batched"]["frames_per_s"] / out["plain"]["frames_per_s"]
+    for d in ds:
+        d.close()
+    return out, pairs
+
+
+def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fracs):
+    """Workload 'served': n_req requests, 95% drawn from the batch scenes and
+    5% of the 1M-triangle 1080p shape, each with a seeded camera pose,
+    admitted against a 200 GB budget.  Reports the plain path (each request
+    rendered alone, one graph replay each), the saturated served rate, and
+    p50/p99 latency at Poisson arrival rates that are fractions of it."""
+    import numpy as np
+    from swegl_amd.device import DeviceScene, Server, Target
+    from swegl_amd.host import SceneDescHolder
+
+    rng = np.random.default_rng(seed)
+    allp = list(pairs) + [big]
+    ds = [DeviceScene(ctx, SceneDescHolder(s)) for s, _ in allp]
+    pools, keep = {}, []
+    slots = lanes * max_batch + 8
+
+    def target(vp, i):
+        key = (vp.m_w, vp.m_h)
+        pool = pools.setdefault(key, [])
+        j = i % (slots if key != (big[1][0].m_w, big[1][0].m_h) else 16)
+        while len(pool) <= j:
+            pool.append((Target(ctx, A.TARGET_COLOR, vp.m_w, vp.m_h),
+                         Target(ctx, A.TARGET_FLOAT, vp.m_w, vp.m_h)))
+        return pool[j]
+
+    descs = (A.ViewportDesc * n_req)()
+    jobs = (A.Job * n_req)()
+    which = np.where(rng.random(n_req) < 0.05, len(pairs), rng.integers(0, len(pairs), n_req))
+    base = {}
+    for i in range(n_req):
+        k = int(which[i])
+        s, vps = allp[k]
+        vp = vps[0]
+        if k not in base:
+            base[k] = vp.m_camera.m_viewmatrix.copy()
+        vp.m_camera.m_viewmatrix[...] = base[k]
+        vp.m_camera.rotate_y(float(rng.uniform(-0.3, 0.3)))
+        col, z = target(vp, i)
+        descs[i] = vp.desc(col, z)
+        jobs[i] = A.Job(ds[k].h, C.pointer(descs[i]), 1)
+    n_big = int((which == len(pairs)).sum())
+
+    # plain: every request alone, in arrival order (graphs captured first)
+    for k in range(len(allp)):
+        i = int(np.argmax(which == k)) if (which == k).any() else None
+        if i is not None:
+            check(L.swegl_render_graph(ctx.h, ds[k].h, C.pointer(descs[i]), 1, None), "warm")
+    ctx.sync()
+    a = time.perf_counter()
+    for i in range(n_req):
+        check(L.swegl_render_graph(ctx.h, ds[int(which[i])].h, C.pointer(descs[i]), 1, None), "plain")
+    ctx.sync()
+    plain_fps = n_req / (time.perf_counter() - a)
+
+    out = {"requests": n_req, "large_requests": n_big, "lanes": lanes, "max_batch": max_batch,
+           "hbm_budget_bytes": 200 * 10**9, "plain_alone": {"frames_per_s": plain_fps}}
+    srv = Server(ctx, 200 * 10**9, max_batch=max_batch, lanes=lanes)
+    # saturated: everything queued at t=0
+    lat = srv.run_trace(jobs, np.zeros(n_req))
+    st = srv.stats()
+    cap_fps = n_req / lat.max()
+    out["saturated"] = {"frames_per_s": cap_fps, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                        "p99_ms": float(np.percentile(lat, 99) * 1e3), "batches": st["batches"]}
+    out["served_over_plain"] = cap_fps / plain_fps
+    rates = []
+    for f in fracs:
+        rate = f * cap_fps
+        arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        b0 = srv.stats()["batches"]
+        lat = srv.run_trace(jobs, arr)
+        el = float(arr[-1] + lat[-1])
+        rates.append({"arrival_per_s": rate, "load": f, "frames_per_s": n_req / max(el, 1e-9),
+                      "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                      "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                      "batches": srv.stats()["batches"] - b0})
+    out["rates"] = rates
+    out["peak_bytes_in_flight"] = srv.stats()["peak_bytes_in_flight"]
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +209,11 @@ def main():
     ap.add_argument("--tile", default=None, help="WxH raster tile, e.g. 32x32")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=256, help="scenes in the batched workload (0 = skip)")
+    ap.add_argument("--batch-iters", type=int, default=10)
+    ap.add_argument("--requests", type=int, default=20000, help="served requests per rate (0 = skip)")
+    ap.add_argument("--lanes", type=int, default=3)
+    ap.add_argument("--max-batch", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -179,6 +322,11 @@ def main():
                      "traffic": traffic, "algorithmic_bytes": dom_bytes},
         "device": info,
     }
+    if args.batch:
+        out["batch"], pairs = bench_batch(ctx, A, check, L, args.batch, args.seed + 1, args.batch_iters)
+        if args.requests:
+            out["served"] = bench_served(ctx, A, check, L, pairs, (scene, vps), args.requests,
+                                         args.seed + 2, args.lanes, args.max_batch, (0.3, 0.6, 0.9))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, vps, args.cpu_seconds)
     print(json.dumps(out))

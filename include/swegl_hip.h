@@ -241,6 +241,43 @@ int  swegl_render_batch(swegl_ctx *ctx, const swegl_job *jobs, uint32_t n_jobs,
 int  swegl_render_batch_graph(swegl_ctx *ctx, const swegl_job *jobs, uint32_t n_jobs,
                               swegl_frame_stats *stats);
 
+/* ---- served: continuous batching ----------------------------------------- */
+/* Requests (a scene plus its viewports) are queued by swegl_server_submit.
+ * swegl_server_poll retires finished batches and admits queued requests into
+ * every free lane as one batched launch sequence, FIFO, as long as the bytes
+ * of the requests in flight stay within the HBM budget.  Batches on
+ * different lanes run concurrently, so requests that arrive while others are
+ * on the GPU start as soon as a lane and memory free up.  Each result is
+ * bit-identical to rendering the request alone. */
+typedef struct swegl_server_config {
+    uint64_t hbm_budget_bytes;   /* admission budget for requests in flight     */
+    uint32_t max_batch;          /* requests per launch sequence (0 = 256)       */
+    uint32_t lanes;              /* batches in flight at once, 1..8 (0 = 2)      */
+} swegl_server_config;
+
+typedef struct swegl_server_stats {
+    uint64_t submitted, completed, batches;
+    uint64_t bytes_in_flight, peak_bytes_in_flight;
+    uint32_t largest_batch;
+} swegl_server_stats;
+
+int  swegl_server_create(swegl_ctx *ctx, const swegl_server_config *cfg, swegl_server **out);
+int  swegl_server_destroy(swegl_server *srv);
+/* Queue one request; its viewport array is copied.  EAGAIN when the request
+ * alone exceeds the budget.  Targets of requests in flight must not be
+ * reused before those requests finish. */
+int  swegl_server_submit(swegl_server *srv, const swegl_job *job, uint64_t *ticket);
+/* Admit and retire; returns up to `cap` finished tickets. */
+int  swegl_server_poll(swegl_server *srv, uint64_t *done, uint32_t cap, uint32_t *n_done);
+/* Block until every submitted request has finished. */
+int  swegl_server_drain(swegl_server *srv);
+int  swegl_server_get_stats(swegl_server *srv, swegl_server_stats *st);
+/* Replay an arrival trace natively: request i is submitted arrival_s[i]
+ * seconds after the call starts (later if a target it writes is still in
+ * flight) and latency_s[i] = completion time - arrival time. */
+int  swegl_server_run_trace(swegl_server *srv, const swegl_job *jobs, const double *arrival_s,
+                            uint32_t n, double *latency_s);
+
 /* ---- host math helpers (same float semantics as the reference) --------- */
 /* matrix44_t::rotate_x/y/z (src/projection/matrix44.cpp:12-67), translate
  * (:87-92), swegl::scale (src/projection/points.cpp:14-30). */

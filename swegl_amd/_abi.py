@@ -93,3 +93,13 @@ def u32ptr(a):
 
 def i32ptr(a):
     return a.ctypes.data_as(I32P)
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32), ("lanes", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
+                ("bytes_in_flight", C.c_uint64), ("peak_bytes_in_flight", C.c_uint64),
+                ("largest_batch", C.c_uint32)]

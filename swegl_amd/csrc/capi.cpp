@@ -6,8 +6,11 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <chrono>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -139,6 +142,21 @@ struct GraphEntry {
     }
 };
 
+// An execution lane: a stream with its own launch resources, upload staging
+// and (for lanes other than the context's) its own per-viewport scratch, so
+// that batches on different lanes can run concurrently.
+struct Lane {
+    hipStream_t stream = nullptr;
+    Resources res;
+    StageSlot stage[4];
+    int stage_next = 0;
+    bool own_views = false;
+    std::map<std::pair<swegl_scene *, int>, std::unique_ptr<ViewScratch>> views;
+    ViewScratch *scratch(swegl_scene *s, int use);
+    int init(bool own);
+    ~Lane();
+};
+
 struct swegl_ctx {
     int device = 0;
     int n_cus = 256;
@@ -146,13 +164,11 @@ struct swegl_ctx {
     int tw = 32, th = 32;
     uint32_t big_tiles = 16; // bin limit (swegl_ctx_set_bin_limit)
     unsigned long long *h_stats = nullptr;   // pinned
-    StageSlot stage[4];
-    int stage_next = 0;
     bool prof = false;
     std::vector<ProfRec> pending;
     std::map<std::string, std::pair<double, uint64_t>> prof_acc;
     std::vector<hipEvent_t> event_pool;
-    Resources res;
+    Lane lane0;   // the context stream
     std::map<std::string, std::unique_ptr<GraphEntry>> graphs;
 };
 
@@ -186,24 +202,24 @@ static void drain_profile(swegl_ctx *c)
 
 // Runs `launch` on the context stream, bracketed by events when profiling.
 template <class F>
-static hipError_t timed(swegl_ctx *c, const char *name, bool capturing, F launch)
+static hipError_t timed(swegl_ctx *c, hipStream_t stream, const char *name, bool capturing, F launch)
 {
     if (!c->prof || capturing)
         return launch();
     ProfRec r{name, take_event(c), take_event(c)};
-    (void)hipEventRecord(r.a, c->stream);
+    (void)hipEventRecord(r.a, stream);
     hipError_t e = launch();
-    (void)hipEventRecord(r.b, c->stream);
+    (void)hipEventRecord(r.b, stream);
     c->pending.push_back(r);
     return e;
 }
 
-// A pinned staging slot for host->device uploads issued on the stream; a
-// slot is reused only after the copy that read it has completed.
-static int stage_acquire(swegl_ctx *c, size_t bytes, StageSlot **out)
+// A pinned staging slot for host->device uploads issued on a lane's stream;
+// a slot is reused only after the copy that read it has completed.
+static int stage_acquire(Lane &L, size_t bytes, StageSlot **out)
 {
-    StageSlot &s = c->stage[c->stage_next];
-    c->stage_next = (c->stage_next + 1) % 4;
+    StageSlot &s = L.stage[L.stage_next];
+    L.stage_next = (L.stage_next + 1) % 4;
     if (s.used)
         HIPCHK(hipEventSynchronize(s.done));
     if (s.cap < bytes) {
@@ -218,19 +234,56 @@ static int stage_acquire(swegl_ctx *c, size_t bytes, StageSlot **out)
     return SWEGL_OK;
 }
 
-static int upload(swegl_ctx *c, void *dst, const void *src, size_t bytes)
+static int upload(Lane &L, void *dst, const void *src, size_t bytes)
 {
     if (!bytes)
         return SWEGL_OK;
     StageSlot *s;
-    int rc = stage_acquire(c, bytes, &s);
+    int rc = stage_acquire(L, bytes, &s);
     if (rc)
         return rc;
     memcpy(s->host, src, bytes);
-    HIPCHK(hipMemcpyAsync(dst, s->host, bytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipEventRecord(s->done, c->stream));
+    HIPCHK(hipMemcpyAsync(dst, s->host, bytes, hipMemcpyHostToDevice, L.stream));
+    HIPCHK(hipEventRecord(s->done, L.stream));
     s->used = true;
     return SWEGL_OK;
+}
+
+ViewScratch *Lane::scratch(swegl_scene *s, int use)
+{
+    if (!own_views) {
+        while ((int)s->views.size() <= use)
+            s->views.emplace_back(new ViewScratch);
+        return s->views[use].get();
+    }
+    std::unique_ptr<ViewScratch> &p = views[{s, use}];
+    if (!p)
+        p.reset(new ViewScratch);
+    return p.get();
+}
+
+int Lane::init(bool own)
+{
+    own_views = own;
+    if (own)
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (StageSlot &st : stage)
+        HIPCHK(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+    return SWEGL_OK;
+}
+
+Lane::~Lane()
+{
+    if (stream)
+        (void)hipStreamSynchronize(stream);
+    for (StageSlot &st : stage) {
+        if (st.host)
+            (void)hipHostFree(st.host);
+        if (st.done)
+            (void)hipEventDestroy(st.done);
+    }
+    if (own_views && stream)
+        (void)hipStreamDestroy(stream);
 }
 
 // ------------------------------------------------------------------ ctx
@@ -254,8 +307,10 @@ extern "C" int swegl_ctx_create(int device, swegl_ctx **out)
     c->n_cus = prop.multiProcessorCount;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc((void **)&c->h_stats, SW_CTR_CONTROL * sizeof(uint32_t), hipHostMallocDefault));
-    for (StageSlot &s : c->stage)
-        HIPCHK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    c->lane0.stream = c->stream;
+    int lrc = c->lane0.init(false);
+    if (lrc)
+        return lrc;
     *out = c.release();
     return SWEGL_OK;
 }
@@ -269,12 +324,6 @@ extern "C" int swegl_ctx_destroy(swegl_ctx *c)
     drain_profile(c);
     for (hipEvent_t e : c->event_pool)
         (void)hipEventDestroy(e);
-    for (StageSlot &s : c->stage) {
-        if (s.host)
-            (void)hipHostFree(s.host);
-        if (s.done)
-            (void)hipEventDestroy(s.done);
-    }
     if (c->h_stats)
         (void)hipHostFree(c->h_stats);
     c->graphs.clear();
@@ -510,9 +559,24 @@ static int scene_pose(swegl_ctx *c, swegl_scene *s, const swegl_scene_desc *d)
     if ((rc = s->nodes.ensure(nodes.size() * sizeof(NodeDev))) ||
         (rc = s->lights.ensure(lights.size() * sizeof(Light4))))
         return rc;
-    if ((rc = upload(c, s->nodes.p, nodes.data(), nodes.size() * sizeof(NodeDev))) ||
-        (rc = upload(c, s->lights.p, lights.data(), lights.size() * sizeof(Light4))))
+    if ((rc = upload(c->lane0, s->nodes.p, nodes.data(), nodes.size() * sizeof(NodeDev))) ||
+        (rc = upload(c->lane0, s->lights.p, lights.data(), lights.size() * sizeof(Light4))))
         return rc;
+    // v_world and normal_world for these poses
+    WorldArgs W;
+    W.px = s->px.as<float>();
+    W.py = s->py.as<float>();
+    W.pz = s->pz.as<float>();
+    W.nx = s->nx.as<float>();
+    W.ny = s->ny.as<float>();
+    W.nz = s->nz.as<float>();
+    W.prims = s->prims.as<PrimDev>();
+    W.nodes = s->nodes.as<NodeDev>();
+    W.n_prims = s->n_prims;
+    W.n_vertices = s->n_vertices;
+    W.world = s->world.as<float4>();
+    W.nworld = s->nworld.as<float4>();
+    HIPCHK(launch_world(W, c->stream));
     return SWEGL_OK;
 }
 
@@ -656,6 +720,8 @@ extern "C" int swegl_scene_update_pose(swegl_ctx *c, swegl_scene *s, const swegl
         if (d->nodes[i].first_primitive != s->h_nodes[i].first_primitive ||
             d->nodes[i].n_primitives != s->h_nodes[i].n_primitives)
             return fail(SWEGL_EINVAL, "pose update changes primitives");
+    // renders on any lane read the scene's node and world buffers
+    HIPCHK(hipDeviceSynchronize());
     return scene_pose(c, s, d);
 }
 
@@ -741,7 +807,7 @@ struct Wave {
     size_t job_offset = 0;   // byte offset of this wave's table in Resources::jobs
 };
 
-static int build_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_wave, Wave &W)
+static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wave &W)
 {
     const uint32_t nj = (uint32_t)jobs.size();
     W.jd.assign(nj, JobDev{});
@@ -750,10 +816,7 @@ static int build_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_w
     for (uint32_t j = 0; j < nj; j++) {
         swegl_scene *s = jobs[j].scene;
         const swegl_viewport_desc &v = *jobs[j].vp;
-        int use = scene_uses[s]++;
-        while ((int)s->views.size() <= use)
-            s->views.emplace_back(new ViewScratch);
-        ViewScratch &vs = *s->views[use];
+        ViewScratch &vs = *L.scratch(s, scene_uses[s]++);
         const size_t V = std::max<uint32_t>(1, s->n_vertices), T = std::max<uint32_t>(1, s->n_tris);
         int rc;
         if ((rc = vs.ndc.ensure(V * 16)) || (rc = vs.yes.ensure(V)) || (rc = vs.rect.ensure(T * 8)) ||
@@ -787,7 +850,6 @@ static int build_wave(swegl_ctx *c, const std::vector<JobIn> &jobs, bool first_w
         J.sun_I = s->sun_I;
         J.world = s->world.as<float4>();
         J.nworld = s->nworld.as<float4>();
-        J.write_world = (first_wave && use == 0) ? 1 : 0;
         J.ndc = vs.ndc.as<float4>();
         J.yes = vs.yes.as<uint8_t>();
         J.rect = vs.rect.as<uint2>();
@@ -864,7 +926,7 @@ static int size_resources(Resources &R, std::vector<Wave> &waves)
 }
 
 // Enqueue the kernels of one wave (no allocation: safe under graph capture).
-static int launch_wave(swegl_ctx *c, Resources &R, const Wave &W, bool first_wave, bool capturing)
+static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool first_wave, bool capturing)
 {
     BatchDev B;
     B.jobs = (const JobDev *)((char *)R.jobs.p + W.job_offset);
@@ -881,10 +943,10 @@ static int launch_wave(swegl_ctx *c, Resources &R, const Wave &W, bool first_wav
     // and big-list lengths by every wave
     const uint32_t zero0 = first_wave ? 0 : SW_CTR_CONTROL;
     const uint32_t groups = (uint32_t)std::max(1, c->n_cus) * 4;
-    hipStream_t st = c->stream;
+    hipStream_t st = L.stream;
 #define STAGE(name, call)                                                              \
     do {                                                                               \
-        hipError_t e_ = timed(c, name, capturing, [&] { return call; });                \
+        hipError_t e_ = timed(c, st, name, capturing, [&] { return call; });            \
         if (e_ != hipSuccess)                                                          \
             return fail(SWEGL_EIO, "%s launch: %s", name, hipGetErrorString(e_));       \
     } while (0)
@@ -900,7 +962,7 @@ static int launch_wave(swegl_ctx *c, Resources &R, const Wave &W, bool first_wav
     return SWEGL_OK;
 }
 
-static int prepare(swegl_ctx *c, const std::vector<JobIn> &all, std::vector<Wave> &waves)
+static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::vector<Wave> &waves)
 {
     HIPCHK(hipSetDevice(c->device));
     for (const JobIn &j : all) {
@@ -928,18 +990,18 @@ static int prepare(swegl_ctx *c, const std::vector<JobIn> &all, std::vector<Wave
         if (groups[w].empty())
             continue;
         waves.emplace_back();
-        int rc = build_wave(c, groups[w], waves.size() == 1, waves.back());
+        int rc = build_wave(c, L, groups[w], waves.back());
         if (rc)
             return rc;
     }
     return SWEGL_OK;
 }
 
-static int read_stats(swegl_ctx *c, Resources &R, swegl_frame_stats *stats)
+static int read_stats(swegl_ctx *c, Lane &L, Resources &R, swegl_frame_stats *stats)
 {
     HIPCHK(hipMemcpyAsync(c->h_stats, R.ctr.p, SW_CTR_CONTROL * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+                          L.stream));
+    HIPCHK(hipStreamSynchronize(L.stream));
     drain_profile(c);
     const unsigned long long *sl = (const unsigned long long *)((const uint32_t *)c->h_stats + SW_CTR_STATS);
     uint64_t sum[8] = {0};
@@ -955,21 +1017,21 @@ static int read_stats(swegl_ctx *c, Resources &R, swegl_frame_stats *stats)
     return SWEGL_OK;
 }
 
-static int render_jobs(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
+static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swegl_frame_stats *stats)
 {
     std::vector<Wave> waves;
-    int rc = prepare(c, all, waves);
+    int rc = prepare(c, L, all, waves);
     if (rc)
         return rc;
-    if ((rc = size_resources(c->res, waves)))
+    if ((rc = size_resources(L.res, waves)))
         return rc;
     for (size_t w = 0; w < waves.size(); w++) {
-        if ((rc = upload(c, (char *)c->res.jobs.p + waves[w].job_offset, waves[w].jd.data(),
+        if ((rc = upload(L, (char *)L.res.jobs.p + waves[w].job_offset, waves[w].jd.data(),
                          waves[w].jd.size() * sizeof(JobDev))) ||
-            (rc = launch_wave(c, c->res, waves[w], w == 0, false)))
+            (rc = launch_wave(c, L, L.res, waves[w], w == 0, false)))
             return rc;
     }
-    return stats ? read_stats(c, c->res, stats) : SWEGL_OK;
+    return stats ? read_stats(c, L, L.res, stats) : SWEGL_OK;
 }
 
 // ----------------------------------------------------------- graph path
@@ -987,7 +1049,7 @@ static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
 static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
 {
     std::vector<Wave> waves;
-    int rc = prepare(c, all, waves);
+    int rc = prepare(c, c->lane0, all, waves);
     if (rc)
         return rc;
     const std::string key = graph_key(c, waves);
@@ -1009,7 +1071,7 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
                                       c->stream);
         int lrc = e == hipSuccess ? SWEGL_OK : fail(SWEGL_EIO, "capture copy: %s", hipGetErrorString(e));
         for (size_t w = 0; lrc == SWEGL_OK && w < waves.size(); w++)
-            lrc = launch_wave(c, g->res, waves[w], w == 0, true);
+            lrc = launch_wave(c, c->lane0, g->res, waves[w], w == 0, true);
         hipGraph_t graph = nullptr;
         hipError_t ee = hipStreamEndCapture(c->stream, &graph);
         if (lrc)
@@ -1048,7 +1110,7 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
     HIPCHK(hipGraphLaunch(G->exec, c->stream));
     HIPCHK(hipEventRecord(G->done[slot], c->stream));
     G->used[slot] = true;
-    return stats ? read_stats(c, G->res, stats) : SWEGL_OK;
+    return stats ? read_stats(c, c->lane0, G->res, stats) : SWEGL_OK;
 }
 
 extern "C" int swegl_render(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vps, uint32_t n,
@@ -1059,7 +1121,7 @@ extern "C" int swegl_render(swegl_ctx *c, swegl_scene *s, const swegl_viewport_d
     std::vector<JobIn> jobs;
     for (uint32_t i = 0; i < n; i++)
         jobs.push_back({s, &vps[i]});
-    return render_jobs(c, jobs, stats);
+    return render_jobs(c, c->lane0, jobs, stats);
 }
 
 extern "C" int swegl_render_graph(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vps,
@@ -1090,7 +1152,7 @@ extern "C" int swegl_render_batch(swegl_ctx *c, const swegl_job *jobs, uint32_t 
         return fail(SWEGL_EINVAL, "null argument");
     std::vector<JobIn> all;
     int rc = collect(jobs, n, all);
-    return rc ? rc : render_jobs(c, all, stats);
+    return rc ? rc : render_jobs(c, c->lane0, all, stats);
 }
 
 extern "C" int swegl_render_batch_graph(swegl_ctx *c, const swegl_job *jobs, uint32_t n,
@@ -1101,6 +1163,266 @@ extern "C" int swegl_render_batch_graph(swegl_ctx *c, const swegl_job *jobs, uin
     std::vector<JobIn> all;
     int rc = collect(jobs, n, all);
     return rc ? rc : render_jobs_graph(c, all, stats);
+}
+
+// ----------------------------------------------------------------- server
+struct SrvReq {
+    uint64_t ticket;
+    swegl_scene *scene;
+    std::vector<swegl_viewport_desc> vps;
+    uint64_t bytes;
+};
+
+struct swegl_server {
+    swegl_ctx *c = nullptr;
+    swegl_server_config cfg{};
+    struct LaneState {
+        std::unique_ptr<Lane> lane;
+        hipEvent_t done = nullptr;
+        bool busy = false;
+        std::vector<SrvReq> reqs;
+        uint64_t bytes = 0;
+    };
+    std::vector<LaneState> lanes;
+    std::deque<SrvReq> pending;
+    std::deque<uint64_t> finished;
+    uint64_t next_ticket = 1, in_flight = 0, peak = 0, submitted = 0, completed = 0, batches = 0;
+    uint32_t largest = 0;
+    ~swegl_server()
+    {
+        for (LaneState &l : lanes) {
+            if (l.lane && l.lane->stream)
+                (void)hipStreamSynchronize(l.lane->stream);
+            if (l.done)
+                (void)hipEventDestroy(l.done);
+        }
+    }
+};
+
+// Device bytes a request holds while in flight: per-viewport scratch, its
+// share of the bin arrays and the surfaces it writes.
+static uint64_t request_bytes(const swegl_scene *s, const std::vector<swegl_viewport_desc> &vps)
+{
+    uint64_t b = 0;
+    for (const swegl_viewport_desc &v : vps) {
+        const uint64_t V = s->n_vertices, T = s->n_tris, P = (uint64_t)v.w * v.h;
+        b += V * 17 + T * 12 + T * 16 * 4 + P * 8;
+        if (v.post_shader != SWEGL_POST_NULL)
+            b += P * 4;
+        if (v.triangle_id)
+            b += P * 4;
+    }
+    return b;
+}
+
+extern "C" int swegl_server_create(swegl_ctx *c, const swegl_server_config *cfg, swegl_server **out)
+{
+    if (!c || !cfg || !out)
+        return fail(SWEGL_EINVAL, "null argument");
+    std::unique_ptr<swegl_server> sv(new swegl_server);
+    sv->c = c;
+    sv->cfg = *cfg;
+    if (!sv->cfg.max_batch)
+        sv->cfg.max_batch = 256;
+    if (!sv->cfg.lanes)
+        sv->cfg.lanes = 2;
+    if (sv->cfg.lanes > 8)
+        return fail(SWEGL_EINVAL, "at most 8 lanes");
+    HIPCHK(hipSetDevice(c->device));
+    sv->lanes.resize(sv->cfg.lanes);
+    for (auto &l : sv->lanes) {
+        l.lane.reset(new Lane);
+        int rc = l.lane->init(true);
+        if (rc)
+            return rc;
+        HIPCHK(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+    }
+    *out = sv.release();
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_server_destroy(swegl_server *sv)
+{
+    delete sv;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_server_submit(swegl_server *sv, const swegl_job *job, uint64_t *ticket)
+{
+    if (!sv || !job || !job->scene || (job->n_viewports && !job->viewports))
+        return fail(SWEGL_EINVAL, "null argument");
+    SrvReq r;
+    r.scene = job->scene;
+    r.vps.assign(job->viewports, job->viewports + job->n_viewports);
+    for (const swegl_viewport_desc &v : r.vps) {
+        int rc = check_viewport(v);
+        if (rc)
+            return rc;
+    }
+    r.bytes = request_bytes(r.scene, r.vps);
+    if (r.bytes > sv->cfg.hbm_budget_bytes)
+        return fail(SWEGL_EAGAIN, "request needs %llu bytes, budget is %llu", (unsigned long long)r.bytes,
+                    (unsigned long long)sv->cfg.hbm_budget_bytes);
+    r.ticket = sv->next_ticket++;
+    if (ticket)
+        *ticket = r.ticket;
+    sv->pending.push_back(std::move(r));
+    sv->submitted++;
+    return SWEGL_OK;
+}
+
+static int server_pump(swegl_server *sv)
+{
+    for (auto &l : sv->lanes) {
+        if (!l.busy)
+            continue;
+        hipError_t q = hipEventQuery(l.done);
+        if (q == hipErrorNotReady)
+            continue;
+        if (q != hipSuccess)
+            return fail(SWEGL_EIO, "batch failed: %s", hipGetErrorString(q));
+        for (const SrvReq &r : l.reqs)
+            sv->finished.push_back(r.ticket);
+        sv->completed += l.reqs.size();
+        sv->in_flight -= l.bytes;
+        l.reqs.clear();
+        l.bytes = 0;
+        l.busy = false;
+    }
+    for (auto &l : sv->lanes) {
+        if (l.busy || sv->pending.empty())
+            continue;
+        uint64_t bytes = 0;
+        while (!sv->pending.empty() && l.reqs.size() < sv->cfg.max_batch &&
+               sv->in_flight + bytes + sv->pending.front().bytes <= sv->cfg.hbm_budget_bytes) {
+            bytes += sv->pending.front().bytes;
+            l.reqs.push_back(std::move(sv->pending.front()));
+            sv->pending.pop_front();
+        }
+        if (l.reqs.empty())
+            break;   // head of the queue waits for memory
+        std::vector<JobIn> all;
+        for (const SrvReq &r : l.reqs)
+            for (const swegl_viewport_desc &v : r.vps)
+                all.push_back({r.scene, &v});
+        int rc = render_jobs(sv->c, *l.lane, all, nullptr);
+        if (rc)
+            return rc;
+        HIPCHK(hipEventRecord(l.done, l.lane->stream));
+        l.busy = true;
+        l.bytes = bytes;
+        sv->in_flight += bytes;
+        sv->peak = std::max(sv->peak, sv->in_flight);
+        sv->batches++;
+        sv->largest = std::max<uint32_t>(sv->largest, (uint32_t)l.reqs.size());
+    }
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_server_poll(swegl_server *sv, uint64_t *done, uint32_t cap, uint32_t *n_done)
+{
+    if (!sv)
+        return fail(SWEGL_EINVAL, "null server");
+    int rc = server_pump(sv);
+    uint32_t n = 0;
+    while (n < cap && !sv->finished.empty()) {
+        if (done)
+            done[n] = sv->finished.front();
+        sv->finished.pop_front();
+        n++;
+    }
+    if (n_done)
+        *n_done = n;
+    return rc;
+}
+
+extern "C" int swegl_server_drain(swegl_server *sv)
+{
+    if (!sv)
+        return fail(SWEGL_EINVAL, "null server");
+    while (sv->completed < sv->submitted) {
+        int rc = server_pump(sv);
+        if (rc)
+            return rc;
+    }
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_server_get_stats(swegl_server *sv, swegl_server_stats *st)
+{
+    if (!sv || !st)
+        return fail(SWEGL_EINVAL, "null argument");
+    st->submitted = sv->submitted;
+    st->completed = sv->completed;
+    st->batches = sv->batches;
+    st->bytes_in_flight = sv->in_flight;
+    st->peak_bytes_in_flight = sv->peak;
+    st->largest_batch = sv->largest;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_server_run_trace(swegl_server *sv, const swegl_job *jobs, const double *arrival_s,
+                                      uint32_t n, double *latency_s)
+{
+    if (!sv || (n && (!jobs || !arrival_s || !latency_s)))
+        return fail(SWEGL_EINVAL, "null argument");
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto now = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    std::unordered_map<uint64_t, uint32_t> index;        // ticket -> request
+    std::unordered_map<const void *, uint64_t> writer;   // target -> ticket in flight
+    std::unordered_map<uint64_t, std::vector<const void *>> writes;
+    uint32_t next = 0, done_n = 0;
+    std::vector<uint64_t> fin(256);
+    while (done_n < n) {
+        const double t = now();
+        while (next < n && arrival_s[next] <= t) {
+            const swegl_job &j = jobs[next];
+            bool busy = false;
+            for (uint32_t k = 0; k < j.n_viewports && !busy; k++) {
+                const void *tg[3] = {j.viewports[k].color, j.viewports[k].depth, j.viewports[k].triangle_id};
+                for (const void *p : tg)
+                    if (p && writer.count(p))
+                        busy = true;
+            }
+            if (busy)
+                break;   // its surfaces are still being written: wait in order
+            uint64_t tk;
+            int rc = swegl_server_submit(sv, &j, &tk);
+            if (rc)
+                return rc;
+            index[tk] = next;
+            for (uint32_t k = 0; k < j.n_viewports; k++) {
+                const void *tg[3] = {j.viewports[k].color, j.viewports[k].depth, j.viewports[k].triangle_id};
+                for (const void *p : tg)
+                    if (p) {
+                        writer[p] = tk;
+                        writes[tk].push_back(p);
+                    }
+            }
+            next++;
+        }
+        uint32_t got = 0;
+        int rc = swegl_server_poll(sv, fin.data(), (uint32_t)fin.size(), &got);
+        if (rc)
+            return rc;
+        if (got) {
+            const double tc = now();
+            for (uint32_t q = 0; q < got; q++) {
+                const uint32_t i = index[fin[q]];
+                latency_s[i] = tc - arrival_s[i];
+                for (const void *p : writes[fin[q]]) {
+                    auto w = writer.find(p);
+                    if (w != writer.end() && w->second == fin[q])
+                        writer.erase(w);
+                }
+                writes.erase(fin[q]);
+                index.erase(fin[q]);
+                done_n++;
+            }
+        }
+    }
+    return SWEGL_OK;
 }
 
 // ----------------------------------------------------- host math exports

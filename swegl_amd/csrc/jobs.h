@@ -9,8 +9,8 @@
 //                                                      the reference's winding)
 //   prims / nodes / texs         : small tables
 //   texels                       : uint32 bitmaps of all textures, concatenated
-//   world, nworld                : float4[n_vertices]  v_world / normal_world of the
-//                                                      current render (written once)
+//   world, nworld                : float4[n_vertices]  v_world / normal_world for the
+//                                                      current node poses
 // Per viewport job:
 //   ndc                          : float4[n_vertices]  frustum coordinates
 //   yes                          : uint8[n_vertices]   visibility marks
@@ -57,8 +57,7 @@ struct JobDev {
     uint32_t n_prims, n_lights, n_vertices, n_tris;
     float ambient, sun_I;
     f3 sun;
-    float4 *world, *nworld;
-    int32_t write_world;
+    const float4 *world, *nworld;
     // ---- per viewport
     float4 *ndc;
     uint8_t *yes;
@@ -80,6 +79,14 @@ struct JobDev {
     uint32_t tile_base;       // first global tile id of this job
     // ---- first block of this job in each batched launch
     uint32_t vblock0, tblock0, dblock0;
+};
+
+struct WorldArgs {
+    const float *px, *py, *pz, *nx, *ny, *nz;
+    const PrimDev *prims;
+    const NodeDev *nodes;
+    uint32_t n_prims, n_vertices;
+    float4 *world, *nworld;
 };
 
 struct BatchDev {

@@ -121,24 +121,41 @@ __global__ void k_clear(BatchDev B, uint32_t w0, uint32_t n_words)
         B.ctr[w0 + i] = 0;
 }
 
+// ------------------------------------------------------------- k_world
+// original_to_world (vertex_shaders.hpp:16-33) and normal_world
+// (:61-63).  Both depend only on the scene and its node poses, so they are
+// computed when a scene is uploaded or re-posed, not per render.
+__global__ __launch_bounds__(256) void k_world(WorldArgs W)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= W.n_vertices)
+        return;
+    int lo = 0, hi = (int)W.n_prims - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (W.prims[mid].first_vertex <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const NodeDev &N = W.nodes[W.prims[lo].node];
+    const f3 w = xform(N.M, f3{W.px[i], W.py[i], W.pz[i]});
+    const f3 n = normal_world(N.RS, f3{W.nx[i], W.ny[i], W.nz[i]});
+    W.world[i] = make_float4(w.x, w.y, w.z, 0.0f);
+    W.nworld[i] = make_float4(n.x, n.y, n.z, 0.0f);
+}
+
 // ------------------------------------------------------------ k_vertex
-// original_to_world (vertex_shaders.hpp:16-33) + world_to_camera_or_frustum
-// (:35-52, 61-71).  v_world is recomputed by every job from the same inputs
-// (bit-identical); only the job flagged write_world stores it.
+// world_to_camera_or_frustum (vertex_shaders.hpp:35-52, 61-71) per viewport:
+// frustum coordinates and cleared visibility marks.
 __global__ __launch_bounds__(256) void k_vertex(BatchDev B)
 {
     const JobDev &J = B.jobs[find_job<JV>(B, blockIdx.x)];
-    uint32_t i = (blockIdx.x - J.vblock0) * 256u + threadIdx.x;
+    const uint32_t i = (blockIdx.x - J.vblock0) * 256u + threadIdx.x;
     if (i >= J.n_vertices)
         return;
-    const NodeDev &N = J.nodes[J.prims[prim_of_vertex(J, i)].node];
-    f3 w = xform(N.M, f3{J.px[i], J.py[i], J.pz[i]});
-    if (J.write_world) {
-        f3 n = normal_world(N.RS, f3{J.nx[i], J.ny[i], J.nz[i]});
-        J.world[i] = make_float4(w.x, w.y, w.z, 0.0f);
-        J.nworld[i] = make_float4(n.x, n.y, n.z, 0.0f);
-    }
-    f3 f = world_to_frustum(J.V, w);
+    const float4 w4 = J.world[i];
+    const f3 f = world_to_frustum(J.V, f3{w4.x, w4.y, w4.z});
     J.ndc[i] = make_float4(f.x, f.y, f.z, 0.0f);
     J.yes[i] = 0;
 }
@@ -809,6 +826,12 @@ hipError_t launch_clear(const BatchDev &B, uint32_t w0, uint32_t n_words, hipStr
 {
     uint32_t n = B.n_tiles > n_words ? B.n_tiles : n_words;
     LAUNCH_CHECK(k_clear, dim3((n + 255) / 256), dim3(256), 0, s, B, w0, n_words);
+    return hipSuccess;
+}
+hipError_t launch_world(const WorldArgs &W, hipStream_t s)
+{
+    if (W.n_vertices)
+        LAUNCH_CHECK(k_world, dim3((W.n_vertices + 255) / 256), dim3(256), 0, s, W);
     return hipSuccess;
 }
 hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, hipStream_t s)

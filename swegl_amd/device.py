@@ -59,6 +59,15 @@ def lib():
                                              C.POINTER(A.FrameStats)]),
             "swegl_render_batch_graph": (C.c_int, [vp, C.POINTER(A.Job), C.c_uint32,
                                                    C.POINTER(A.FrameStats)]),
+            "swegl_server_create": (C.c_int, [vp, C.POINTER(A.ServerConfig), C.POINTER(vp)]),
+            "swegl_server_destroy": (C.c_int, [vp]),
+            "swegl_server_submit": (C.c_int, [vp, C.POINTER(A.Job), C.POINTER(C.c_uint64)]),
+            "swegl_server_poll": (C.c_int, [vp, C.POINTER(C.c_uint64), C.c_uint32,
+                                            C.POINTER(C.c_uint32)]),
+            "swegl_server_drain": (C.c_int, [vp]),
+            "swegl_server_get_stats": (C.c_int, [vp, C.POINTER(A.ServerStats)]),
+            "swegl_server_run_trace": (C.c_int, [vp, C.POINTER(A.Job), C.POINTER(C.c_double),
+                                                 C.c_uint32, C.POINTER(C.c_double)]),
             "swegl_matrix_rotate_x": (None, [A.FP, C.c_float]),
             "swegl_matrix_rotate_y": (None, [A.FP, C.c_float]),
             "swegl_matrix_rotate_z": (None, [A.FP, C.c_float]),
@@ -206,6 +215,56 @@ class DeviceScene:
         if self.h and self.ctx.h:
             lib().swegl_scene_destroy(self.ctx.h, self.h)
         self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """swegl_server: continuous batching of render requests over lanes."""
+
+    def __init__(self, ctx, budget_bytes, max_batch=256, lanes=2):
+        h = C.c_void_p()
+        cfg = A.ServerConfig(int(budget_bytes), int(max_batch), int(lanes))
+        check(lib().swegl_server_create(ctx.h, C.byref(cfg), C.byref(h)), "swegl_server_create")
+        self.ctx, self.h = ctx, h
+
+    def submit(self, job):
+        t = C.c_uint64()
+        check(lib().swegl_server_submit(self.h, C.byref(job), C.byref(t)), "swegl_server_submit")
+        return t.value
+
+    def poll(self, cap=1024):
+        buf = (C.c_uint64 * cap)()
+        n = C.c_uint32()
+        check(lib().swegl_server_poll(self.h, buf, cap, C.byref(n)), "swegl_server_poll")
+        return list(buf[:n.value])
+
+    def drain(self):
+        check(lib().swegl_server_drain(self.h), "swegl_server_drain")
+
+    def stats(self):
+        st = A.ServerStats()
+        check(lib().swegl_server_get_stats(self.h, C.byref(st)), "swegl_server_get_stats")
+        return {k: getattr(st, k) for k, _ in A.ServerStats._fields_}
+
+    def run_trace(self, jobs, arrival_s):
+        import numpy as np
+        n = len(arrival_s)
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        lat = np.zeros(n, np.float64)
+        check(lib().swegl_server_run_trace(self.h, jobs, arr.ctypes.data_as(C.POINTER(C.c_double)), n,
+                                           lat.ctypes.data_as(C.POINTER(C.c_double))),
+              "swegl_server_run_trace")
+        return lat
+
+    def close(self):
+        if self.h:
+            lib().swegl_server_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
