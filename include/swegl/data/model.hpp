@@ -1,0 +1,138 @@
+// swegl/data/model.hpp -- scene description of the swegl API (reference:
+// swegl/data/model.hpp, swegl/data/texture.hpp), kept in the reference's
+// array-of-structures form.  The GPU render does not write the per-vertex
+// scratch fields (v_world, v_viewport, normal_world, yes): they stay for
+// layout compatibility only.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "../projection/points.hpp"
+#include "../render/colors.hpp"
+
+namespace swegl
+{
+
+using vertex_idx = std::uint32_t;
+
+struct mesh_vertex_t {
+    vertex_t v;
+    vertex_t v_world;
+    vertex_t v_viewport;
+    vec2f_t tex_coords;
+    normal_t normal;
+    normal_t normal_world;
+    bool yes = false;
+};
+
+struct primitive_t {
+    enum index_mode_t {
+        POINTS = 0,
+        LINES = 1,
+        LINE_LOOP = 2,
+        LINE_STRIP = 3,
+        TRIANGLES = 4,
+        TRIANGLE_STRIP = 5,
+        TRIANGLE_FAN = 6,
+    };
+    std::vector<mesh_vertex_t> vertices;
+    std::vector<vertex_idx> indices;
+    index_mode_t mode;
+    int material_id;
+};
+
+struct node_t {
+    vertex_t scale = vertex_t(1, 1, 1);
+    matrix44_t rotation = matrix44_t::identity();
+    vertex_t translation = vertex_t(0, 0, 0);
+    std::vector<primitive_t> primitives;
+    std::vector<int> children_idx;
+    bool root = true;
+};
+
+struct point_source_light {
+    vertex_t position;
+    float intensity;
+};
+
+struct material_t {
+    pixel_colors color = pixel_colors(255, 255, 255, 255);
+    float metallic = 1;
+    float roughness = 1;
+    int texture_idx = -1;
+    bool double_sided = false;
+};
+
+// mip 0 of a texture_t (texture.hpp:12-38)
+struct mipmap_t {
+    std::vector<unsigned int> m_bitmap;
+    unsigned int m_width = 0, m_height = 0;
+};
+
+class texture_t
+{
+public:
+    std::vector<std::shared_ptr<mipmap_t>> m_mipmaps;
+    texture_t(const unsigned int *data, int w, int h)
+    {
+        auto m = std::make_shared<mipmap_t>();
+        m->m_bitmap.assign(data, data + (size_t)w * h);
+        m->m_width = (unsigned)w;
+        m->m_height = (unsigned)h;
+        m_mipmaps.push_back(m);
+    }
+};
+
+struct scene_t {
+    std::vector<node_t> nodes;
+    std::vector<int> root_nodes;
+    float ambient_light_intensity = 0;
+    normal_t sun_direction = normal_t(0, -1, 0);
+    float sun_intensity = 0;
+    material_t default_material;
+    std::vector<point_source_light> point_source_lights;
+    std::vector<material_t> materials;
+    std::vector<texture_t> images;
+    // Bump after editing vertices, indices or textures in place: render()
+    // re-uploads meshes and textures only when this changes.
+    std::uint64_t device_generation = 0;
+};
+
+// make_cube (model.hpp:254-325): six 4-vertex triangle fans.
+inline node_t make_cube(float size, int material_idx)
+{
+    struct face {
+        float p[4][3];
+        float t[4][2];
+        float n[3];
+    };
+    const float h = size / 2.0f;
+    const face faces[6] = {
+        {{{-h, h, h}, {h, h, h}, {h, h, -h}, {-h, h, -h}}, {{0.0, 0.0}, {0.5, 0.0}, {0.5, 0.333}, {0.0, 0.333}}, {0, 1, 0}},
+        {{{-h, -h, h}, {h, -h, h}, {h, h, h}, {-h, h, h}}, {{0.5, 0.0}, {1.0, 0.0}, {1.0, 0.333}, {0.5, 0.333}}, {0, 0, 1}},
+        {{{h, -h, h}, {h, -h, -h}, {h, h, -h}, {h, h, h}}, {{0.0, 0.333}, {0.5, 0.333}, {0.5, 0.667}, {0.0, 0.667}}, {1, 0, 0}},
+        {{{h, -h, -h}, {-h, -h, -h}, {-h, h, -h}, {h, h, -h}}, {{0.5, 0.667}, {1.0, 0.667}, {1.0, 1.0}, {0.5, 1.0}}, {0, 0, -1}},
+        {{{-h, -h, -h}, {-h, -h, h}, {-h, h, h}, {-h, h, -h}}, {{0.5, 0.333}, {1.0, 0.333}, {1.0, 0.667}, {0.5, 0.667}}, {-1, 0, 0}},
+        {{{-h, -h, -h}, {h, -h, -h}, {h, -h, h}, {-h, -h, h}}, {{0.0, 0.667}, {0.5, 0.667}, {0.5, 1.0}, {0.0, 1.0}}, {0, -1, 0}},
+    };
+    node_t node;
+    for (const face &f : faces) {
+        primitive_t p;
+        for (int i = 0; i < 4; i++) {
+            mesh_vertex_t mv;
+            mv.v = vertex_t(f.p[i][0], f.p[i][1], f.p[i][2]);
+            mv.tex_coords = vec2f_t(f.t[i][0], f.t[i][1]);
+            mv.normal = normal_t(f.n[0], f.n[1], f.n[2]);
+            p.vertices.push_back(mv);
+        }
+        p.indices = {0, 1, 2, 3};
+        p.mode = primitive_t::TRIANGLE_FAN;
+        p.material_id = material_idx;
+        node.primitives.push_back(p);
+    }
+    return node;
+}
+
+}  // namespace swegl

@@ -199,6 +199,10 @@ int  swegl_target_destroy(swegl_ctx *ctx, swegl_target *t);
  * in bytes (SDL_Surface::pitch). */
 int  swegl_target_read(swegl_ctx *ctx, swegl_target *t, void *dst, int32_t dst_pitch);
 int  swegl_target_write(swegl_ctx *ctx, swegl_target *t, const void *src, int32_t src_pitch);
+/* Blocking copy of the rectangle [x, x+w) x [y, y+h) of a target to host
+ * memory (dst points at the rectangle's first element, dst_pitch in bytes). */
+int  swegl_target_read_rect(swegl_ctx *ctx, swegl_target *t, int32_t x, int32_t y, int32_t w,
+                            int32_t h, void *dst, int32_t dst_pitch);
 /* Enqueue a fill of every element with a 32-bit pattern. */
 int  swegl_target_fill(swegl_ctx *ctx, swegl_target *t, uint32_t word);
 

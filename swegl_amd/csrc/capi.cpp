@@ -467,6 +467,20 @@ extern "C" int swegl_target_read(swegl_ctx *c, swegl_target *t, void *dst, int32
     return SWEGL_OK;
 }
 
+extern "C" int swegl_target_read_rect(swegl_ctx *c, swegl_target *t, int32_t x, int32_t y, int32_t w,
+                                      int32_t h, void *dst, int32_t pitch)
+{
+    if (!c || !t || !dst || w <= 0 || h <= 0 || x < 0 || y < 0 || x + w > t->w || y + h > t->h ||
+        pitch < w * 4)
+        return fail(SWEGL_EINVAL, "bad read_rect arguments");
+    const char *src = (const char *)t->buf.p + ((size_t)y * t->w + x) * 4;
+    HIPCHK(hipMemcpy2DAsync(dst, pitch, src, (size_t)t->w * 4, (size_t)w * 4, h, hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_profile(c);
+    return SWEGL_OK;
+}
+
 extern "C" int swegl_target_write(swegl_ctx *c, swegl_target *t, const void *src, int32_t pitch)
 {
     if (!c || !t || !src || pitch < t->w * 4)
