@@ -178,6 +178,8 @@ int  swegl_ctx_get_tile(swegl_ctx *ctx, int *tile_w, int *tile_h);
  * is not binned per tile but listed once per viewport and tested by every
  * tile (default 16; 0 sends every triangle through that list). */
 int  swegl_ctx_set_bin_limit(swegl_ctx *ctx, uint32_t tiles);
+/* Persistent tile-raster workgroups per compute unit (1..16, default 4). */
+int  swegl_ctx_set_raster_groups(swegl_ctx *ctx, uint32_t per_cu);
 
 /* Per-kernel HIP-event timing on the context stream (0 = off). */
 int  swegl_ctx_set_profiling(swegl_ctx *ctx, int on);

@@ -163,6 +163,7 @@ struct swegl_ctx {
     hipStream_t stream = nullptr;
     int tw = 32, th = 32;
     uint32_t big_tiles = 16; // bin limit (swegl_ctx_set_bin_limit)
+    uint32_t groups_per_cu = 4;   // persistent raster workgroups per CU
     unsigned long long *h_stats = nullptr;   // pinned
     bool prof = false;
     std::vector<ProfRec> pending;
@@ -388,6 +389,16 @@ extern "C" int swegl_ctx_set_bin_limit(swegl_ctx *c, uint32_t tiles)
     if (tiles > 4096)
         return fail(SWEGL_EINVAL, "bin limit %u above 4096", tiles);
     c->big_tiles = tiles;
+    return SWEGL_OK;
+}
+
+extern "C" int swegl_ctx_set_raster_groups(swegl_ctx *c, uint32_t per_cu)
+{
+    if (!c)
+        return fail(SWEGL_EINVAL, "null ctx");
+    if (per_cu < 1 || per_cu > 16)
+        return fail(SWEGL_EINVAL, "raster groups per CU %u outside [1,16]", per_cu);
+    c->groups_per_cu = per_cu;
     return SWEGL_OK;
 }
 
@@ -956,7 +967,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     // statistics are zeroed once per render by the first wave; control words
     // and big-list lengths by every wave
     const uint32_t zero0 = first_wave ? 0 : SW_CTR_CONTROL;
-    const uint32_t groups = (uint32_t)std::max(1, c->n_cus) * 4;
+    const uint32_t groups = (uint32_t)std::max(1, c->n_cus) * c->groups_per_cu;
     hipStream_t st = L.stream;
 #define STAGE(name, call)                                                              \
     do {                                                                               \
@@ -1052,7 +1063,8 @@ static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swe
 
 static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
 {
-    std::string k = std::to_string(c->tw) + "x" + std::to_string(c->th) + "/" + std::to_string(c->big_tiles);
+    std::string k = std::to_string(c->tw) + "x" + std::to_string(c->th) + "/" + std::to_string(c->big_tiles) +
+                    "/" + std::to_string(c->groups_per_cu);
     for (const Wave &W : waves) {
         k += "|" + std::to_string(W.jd.size()) + ":" + std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
              "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap);

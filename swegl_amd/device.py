@@ -39,6 +39,7 @@ def lib():
             "swegl_ctx_get_tile": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "swegl_ctx_set_profiling": (C.c_int, [vp, C.c_int]),
             "swegl_ctx_set_bin_limit": (C.c_int, [vp, C.c_uint32]),
+            "swegl_ctx_set_raster_groups": (C.c_int, [vp, C.c_uint32]),
             "swegl_ctx_kernel_times": (C.c_int, [vp, C.c_char_p, C.c_int, C.POINTER(C.c_double),
                                                  C.POINTER(C.c_uint64), C.c_int,
                                                  C.POINTER(C.c_int)]),
@@ -138,6 +139,16 @@ class Context:
 
     def set_bin_limit(self, tiles):
         check(lib().swegl_ctx_set_bin_limit(self.h, int(tiles)), "swegl_ctx_set_bin_limit")
+
+    def set_raster_groups(self, per_cu):
+        check(lib().swegl_ctx_set_raster_groups(self.h, int(per_cu)), "swegl_ctx_set_raster_groups")
+
+    def apply_tuning(self, t):
+        """Apply one entry of swegl_amd/tuning.json: {"tile": [w, h],
+        "bin_limit": n, "raster_groups": g}."""
+        self.set_tile(*t["tile"])
+        self.set_bin_limit(t["bin_limit"])
+        self.set_raster_groups(t["raster_groups"])
 
     def set_profiling(self, on):
         check(lib().swegl_ctx_set_profiling(self.h, int(on)), "swegl_ctx_set_profiling")
