@@ -39,58 +39,112 @@ def algorithmic_bytes(scene, vp, f_vis):
     return frame, raster
 
 
-def cpu_baseline(scene, vps, budget_s):
-    import oracle
-    oracle.render(scene, vps)          # warm (page in, build caches)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        oracle.render(scene, vps)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 200:
-            break
+def _cpu_model():
     cpu = platform.processor() or platform.machine()
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": n / el, "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": f"{n} frames of the same workload, single thread, {el:.1f} s ({cpu})"}
+    return cpu
 
 
-def bench_batch(ctx, A, check, L, count, seed, iters):
-    """Workload 'batch': `count` height fields (T log-uniform in [1e3, 2e5],
-    square framebuffers of 256..1024, DoF on half).  Plain = one graph
-    replay per scene; batched = one graph replay for the whole set."""
+def cpu_baseline(scene, vps, budget_s, threads):
+    """The CPU oracle on this host: one frame per thread on `threads` threads
+    (throughput, the figure reported), and single-threaded frames (the
+    reference renders a frame on one thread)."""
+    import concurrent.futures as cf
+
+    import oracle
+    oracle.render(scene, vps)          # warm (page in, build caches)
+    n1, t0 = 0, time.perf_counter()
+    while True:
+        oracle.render(scene, vps)
+        n1 += 1
+        el1 = time.perf_counter() - t0
+        if el1 >= budget_s / 2 or n1 >= 200:
+            break
+    # oracle_render runs without the GIL (ctypes), so threads render in parallel
+    stop = time.perf_counter() + budget_s / 2
+
+    def worker():
+        k = 0
+        while time.perf_counter() < stop:
+            oracle.render(scene, vps)
+            k += 1
+        return k
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        nt = sum(f.result() for f in [ex.submit(worker) for _ in range(threads)])
+    elt = time.perf_counter() - t0
+    cpu = _cpu_model()
+    return {"value": nt / elt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{nt} frames of the same workload, one frame per thread on {threads} threads, "
+                      f"{elt:.1f} s ({cpu})",
+            "single_thread": {"frames_per_s": n1 / el1, "ms_per_frame": el1 * 1e3 / n1,
+                              "sample": f"{n1} frames, {el1:.1f} s"}}
+
+
+def fp64_reshade(ctx, A, check, L, ds, scene, vp):
+    """The frame's FP32 colours (post shader off) against the oracle's FP64
+    re-shade of the same winners; depth and triangle ids must agree."""
     import numpy as np
-    from swegl_amd import scenes
+
+    import oracle
+    from swegl_amd import host
+    from swegl_amd.device import Target
+
+    keep = vp.m_post_shader
+    vp.set_post_shader(host.post_shader_t())
+    try:
+        col = Target(ctx, A.TARGET_COLOR, vp.m_screen.w, vp.m_screen.h)
+        z = Target(ctx, A.TARGET_FLOAT, vp.m_w, vp.m_h)
+        tid = Target(ctx, A.TARGET_UINT, vp.m_w, vp.m_h)
+        d = (A.ViewportDesc * 1)()
+        d[0] = vp.desc(col, z, tid)
+        check(L.swegl_render(ctx.h, ds.h, d, 1, None), "swegl_render")
+        ctx.sync()
+        want = oracle.render(scene, [vp], reshade64=True)[0]
+        g_tid, g_z = tid.read(), z.read()
+        same = bool(np.array_equal(g_tid, want.tid) and
+                    np.array_equal(g_z.view(np.uint32), want.zbuffer.view(np.uint32)))
+        px = col.read().reshape(vp.m_screen.h, vp.m_screen.w)[vp.m_y:vp.m_y + vp.m_h, vp.m_x:vp.m_x + vp.m_w]
+        e = oracle.reshade_error(px, g_tid, want.color64)
+        e["depth_and_ids_bit_exact"] = same
+        return e
+    finally:
+        vp.set_post_shader(keep)
+
+
+def plain_vs_batched(ctx, A, check, L, pairs, iters):
+    """Render every (scene, viewports) pair resident in HBM two ways:
+    plain = one graph replay per scene (all its viewports), batched = one
+    graph replay for the whole set.  A frame is one viewport."""
     from swegl_amd.device import DeviceScene, Target
     from swegl_amd.host import SceneDescHolder
 
-    t0 = time.perf_counter()
-    pairs = scenes.batch_scenes(seed, count=count)
-    gen_s = time.perf_counter() - t0
+    count = len(pairs)
     ds = [DeviceScene(ctx, SceneDescHolder(s)) for s, _ in pairs]
     keep, descs = [], []
     for s, vps in pairs:
-        vp = vps[0]
-        col = Target(ctx, A.TARGET_COLOR, vp.m_screen.w, vp.m_screen.h)
-        z = Target(ctx, A.TARGET_FLOAT, vp.m_w, vp.m_h)
-        keep += [col, z]
-        arr = (A.ViewportDesc * 1)()
-        arr[0] = vp.desc(col, z)
+        arr = (A.ViewportDesc * len(vps))()
+        for i, vp in enumerate(vps):
+            col = Target(ctx, A.TARGET_COLOR, vp.m_screen.w, vp.m_screen.h)
+            z = Target(ctx, A.TARGET_FLOAT, vp.m_w, vp.m_h)
+            keep += [col, z]
+            arr[i] = vp.desc(col, z)
         descs.append(arr)
     jobs = (A.Job * count)()
     for k in range(count):
-        jobs[k] = A.Job(ds[k].h, descs[k], 1)
+        jobs[k] = A.Job(ds[k].h, descs[k], len(descs[k]))
+    n_frames = sum(len(d) for d in descs)
 
     def plain():
         for k in range(count):
-            check(L.swegl_render_graph(ctx.h, ds[k].h, descs[k], 1, None), "render_graph")
+            check(L.swegl_render_graph(ctx.h, ds[k].h, descs[k], len(descs[k]), None), "render_graph")
 
     def batched():
         check(L.swegl_render_batch_graph(ctx.h, jobs, count, None), "render_batch_graph")
@@ -98,9 +152,8 @@ def bench_batch(ctx, A, check, L, count, seed, iters):
     st = (A.FrameStats * 1)()
     check(L.swegl_render_batch(ctx.h, jobs, count, st), "render_batch")
     frags = st[0].fragments
-    out = {"scenes": count, "triangles": int(sum(s.n_triangles() for s, _ in pairs)),
-           "pixels": int(sum(v[0].m_w * v[0].m_h for _, v in pairs)), "fragments": int(frags),
-           "generate_s": round(gen_s, 1)}
+    out = {"scenes": count, "frames": n_frames, "triangles": int(sum(s.n_triangles() for s, _ in pairs)),
+           "pixels": int(sum(vp.m_w * vp.m_h for _, v in pairs for vp in v)), "fragments": int(frags)}
     for name, fn in (("plain", plain), ("batched", batched)):
         fn()
         fn()
@@ -110,13 +163,41 @@ def bench_batch(ctx, A, check, L, count, seed, iters):
             fn()
         ctx.sync()
         el = time.perf_counter() - a
-        fps = count * iters / el
-        out[name] = {"frames_per_s": fps, "gfragments_per_s": frags * iters / el / 1e9,
+        out[name] = {"frames_per_s": n_frames * iters / el, "gfragments_per_s": frags * iters / el / 1e9,
                      "ms_per_batch": el * 1e3 / iters}
     out["batched_over_plain"] = out["batched"]["frames_per_s"] / out["plain"]["frames_per_s"]
     for d in ds:
         d.close()
+    return out
+
+
+def bench_batch(ctx, A, check, L, count, seed, iters):
+    """Workload 'batch': `count` height fields (T log-uniform in [1e3, 2e5],
+    square framebuffers of 256..1024, DoF on half)."""
+    from swegl_amd import scenes
+
+    t0 = time.perf_counter()
+    pairs = scenes.batch_scenes(seed, count=count)
+    gen_s = time.perf_counter() - t0
+    out = plain_vs_batched(ctx, A, check, L, pairs, iters)
+    out["generate_s"] = round(gen_s, 1)
     return out, pairs
+
+
+def bench_heavy(ctx, A, check, L, count, seed, iters):
+    """Workload 'heavy': `count` scenes of 500 stacked 1000-triangle
+    clusters (500,000 triangles), 4096^2 texture, sun + 8 point lights,
+    each seen by 4 viewports of 1920x1080 with the 9-tap DoF."""
+    from swegl_amd import scenes
+
+    t0 = time.perf_counter()
+    pairs = scenes.heavy_scenes(seed, count=count)
+    gen_s = time.perf_counter() - t0
+    out = plain_vs_batched(ctx, A, check, L, pairs, iters)
+    out["generate_s"] = round(gen_s, 1)
+    out["workload"] = (f"{count} scenes x 500,000 triangles (500 clusters of 1000, stacks of 8-20), "
+                       "4 x 1920x1080 viewports each, tex 4096^2, sun + 8 point lights, 9-tap DoF r_max 8")
+    return out
 
 
 def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fracs):
@@ -208,12 +289,17 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tile", default=None, help="WxH raster tile, e.g. 32x32")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=256, help="scenes in the batched workload (0 = skip)")
     ap.add_argument("--batch-iters", type=int, default=10)
+    ap.add_argument("--heavy", type=int, default=64, help="scenes in the heavy workload (0 = skip)")
+    ap.add_argument("--heavy-iters", type=int, default=3)
     ap.add_argument("--requests", type=int, default=20000, help="served requests per rate (0 = skip)")
     ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--max-batch", type=int, default=64)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also copy a target of this many MiB 3 times (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -229,6 +315,10 @@ def main():
     scene, vps = scenes.heightfield_scene(args.seed)
     vp = vps[0]
     ctx = Context(0)
+    from swegl_amd import autotune
+    tuning = autotune.load()
+    if "heightfield" in tuning:
+        ctx.apply_tuning(tuning["heightfield"])
     if args.tile:
         tw, th = (int(x) for x in args.tile.lower().split("x"))
         ctx.set_tile(tw, th)
@@ -243,6 +333,12 @@ def main():
     def frame(stats=None):
         check(L.swegl_render(ctx.h, ds.h, desc, 1, stats), "swegl_render")
 
+    if args.calibrate_copy:
+        side = int((args.calibrate_copy * 1024 * 1024 // 4) ** 0.5)
+        ca, cb = Target(ctx, A.TARGET_UINT, side, side), Target(ctx, A.TARGET_UINT, side, side)
+        for _ in range(3):
+            check(L.swegl_target_copy(ctx.h, cb.h, ca.h), "swegl_target_copy")
+        ctx.sync()
     st = (A.FrameStats * 1)()
     frame(st)
     stats = {k: getattr(st[0], k) for k, _ in A.FrameStats._fields_}
@@ -284,7 +380,7 @@ def main():
     dom_bytes = raster_bytes if dom[0] == "raster" else None
     achieved = dom_bytes / (dom[1][0] * 1e-3) / 1e9 if dom_bytes else None
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_heightfield_1080p.json")
+    tfile = os.path.join(ROOT, "profiles", "heightfield_1080p_kernels.json")
     if os.path.exists(tfile):
         try:
             with open(tfile) as f:
@@ -308,7 +404,9 @@ def main():
         "data": "synthetic (seeded height field, random texture)",
         "config": {"workload": "heightfield 999,698 triangles, 1920x1080, tex 2048^2, "
                                "sun + 4 point lights, 9-tap DoF r_max 8",
-                   "path": "plain", "tile": "%dx%d" % ctx.get_tile(), "seed": args.seed},
+                   "path": "plain", "tile": "%dx%d" % ctx.get_tile(), "seed": args.seed,
+                   "tuning": tuning.get("heightfield") and {k: tuning["heightfield"][k] for k in
+                                                            ("tile", "bin_limit", "raster_groups")}},
         "gfragments_per_s": stats["fragments"] * fps / 1e9,
         "fragments_per_frame": stats["fragments"],
         "shaded_pixels": f_vis,
@@ -323,12 +421,19 @@ def main():
         "device": info,
     }
     if args.batch:
+        if "batch" in tuning:
+            ctx.apply_tuning(tuning["batch"])
         out["batch"], pairs = bench_batch(ctx, A, check, L, args.batch, args.seed + 1, args.batch_iters)
         if args.requests:
             out["served"] = bench_served(ctx, A, check, L, pairs, (scene, vps), args.requests,
                                          args.seed + 2, args.lanes, args.max_batch, (0.3, 0.6, 0.9))
+    if args.heavy:
+        if "heavy" in tuning:
+            ctx.apply_tuning(tuning["heavy"])
+        out["heavy"] = bench_heavy(ctx, A, check, L, args.heavy, args.seed + 3, args.heavy_iters)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, vps, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(scene, vps, args.cpu_seconds, args.cpu_threads)
+        out["fp64_reshade"] = fp64_reshade(ctx, A, check, L, ds, scene, vp)
     print(json.dumps(out))
 
 

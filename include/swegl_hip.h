@@ -205,6 +205,8 @@ int  swegl_target_write(swegl_ctx *ctx, swegl_target *t, const void *src, int32_
  * memory (dst points at the rectangle's first element, dst_pitch in bytes). */
 int  swegl_target_read_rect(swegl_ctx *ctx, swegl_target *t, int32_t x, int32_t y, int32_t w,
                             int32_t h, void *dst, int32_t dst_pitch);
+/* Enqueue a copy of one target into another of the same size and kind. */
+int  swegl_target_copy(swegl_ctx *ctx, swegl_target *dst, swegl_target *src);
 /* Enqueue a fill of every element with a 32-bit pattern. */
 int  swegl_target_fill(swegl_ctx *ctx, swegl_target *t, uint32_t word);
 

@@ -24,7 +24,7 @@ class OracleViewport(C.Structure):
                 ("view", C.c_float * 16), ("projection", C.c_float * 16),
                 ("camera_position", C.c_float * 3), ("post_shader", C.c_int32),
                 ("focal_distance", C.c_float), ("focal_depth", C.c_float),
-                ("dof_max_radius", C.c_int32)]
+                ("dof_max_radius", C.c_int32), ("color64", A.U32P)]
 
 
 class OracleStats(C.Structure):
@@ -69,13 +69,16 @@ def lib():
 class Frame:
     """Oracle output of one viewport: surface, zbuffer (h, w), triangle ids."""
 
-    def __init__(self, pixels, zbuffer, tid, stats):
+    def __init__(self, pixels, zbuffer, tid, stats, color64=None):
         self.pixels, self.zbuffer, self.tid, self.stats = pixels, zbuffer, tid, stats
+        self.color64 = color64
 
 
-def render(scene, viewports, surfaces=None):
+def render(scene, viewports, surfaces=None, reshade64=False):
     """swegl::render(scene, vp...) on the CPU oracle.  Each viewport renders
-    into a copy of its surface's current pixels (or `surfaces[i]`)."""
+    into a copy of its surface's current pixels (or `surfaces[i]`).  With
+    `reshade64` each frame also carries `color64` (h, w): the FP64 re-shade
+    of every pixel's final winner, before the post shader."""
     holder = SceneDescHolder(scene)
     n = len(viewports)
     ovs = (OracleViewport * n)()
@@ -92,13 +95,15 @@ def render(scene, viewports, surfaces=None):
             pix = shared[key]
         z = np.zeros((vp.m_h, vp.m_w), np.float32)
         tid = np.zeros((vp.m_h, vp.m_w), np.uint32)
-        keep += [pix, z, tid]
+        c64 = np.zeros((vp.m_h, vp.m_w), np.uint32) if reshade64 else None
+        keep += [pix, z, tid, c64]
         o = ovs[i]
         o.x, o.y, o.w, o.h = vp.m_x, vp.m_y, vp.m_w, vp.m_h
         o.surface = A.u32ptr(pix)
         o.surface_w, o.surface_h, o.pitch = s.w, s.h, s.pitch
         o.zbuffer = A.fptr(z)
         o.triangle_id = A.u32ptr(tid)
+        o.color64 = A.u32ptr(c64) if reshade64 else None
         o.view[:] = [float(v) for v in vp.m_camera.m_viewmatrix.reshape(16)]
         o.projection[:] = [float(v) for v in vp.m_camera.m_projectionmatrix.reshape(16)]
         o.camera_position[:] = [float(v) for v in vp.m_camera.m_center]
@@ -114,5 +119,20 @@ def render(scene, viewports, surfaces=None):
     frames = []
     for i in range(n):
         d = {k: getattr(st[i], k) for k, _ in OracleStats._fields_}
-        frames.append(Frame(keep[3 * i], keep[3 * i + 1], keep[3 * i + 2], d))
+        frames.append(Frame(keep[4 * i], keep[4 * i + 1], keep[4 * i + 2], d, keep[4 * i + 3]))
     return frames
+
+
+def reshade_error(color, tid, color64):
+    """Max and mean absolute difference (8-bit LSB, over the B, G and R
+    channels of every pixel some triangle drew) between FP32-shaded colours
+    `color` (h, w) and the FP64 re-shade `color64` of the same winners."""
+    drawn = tid != 0xFFFFFFFF
+    n = int(drawn.sum())
+    if n == 0:
+        return {"max": 0, "mean": 0.0, "pixels": 0}
+    a = color[drawn].view(np.uint8).reshape(-1, 4)[:, :3].astype(np.int16)
+    b = color64[drawn].view(np.uint8).reshape(-1, 4)[:, :3].astype(np.int16)
+    d = np.abs(a - b)
+    return {"max": int(d.max()), "mean": float(d.mean()), "pixels": n,
+            "frac_exact": float((d.max(axis=1) == 0).mean())}

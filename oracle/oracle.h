@@ -29,6 +29,9 @@ typedef struct oracle_viewport {
     int32_t   post_shader;             /* SWEGL_POST_*                        */
     float     focal_distance, focal_depth;
     int32_t   dof_max_radius;
+    uint32_t *color64;                 /* w*h or NULL: FP64 re-shade of each  */
+                                       /* pixel's final winner, before the    */
+                                       /* post shader; 0 where nothing drew   */
 } oracle_viewport;
 
 typedef struct oracle_stats {

@@ -219,6 +219,7 @@ struct viewport_m {
     int pitch_words;
     float *zb;
     uint32_t *tid;
+    uint32_t *color64 = nullptr;   /* w*h FP64 re-shade, or NULL */
     const float *view, *proj;
     vec3 cam;
     float m00, m03, m11, m13;   /* viewport matrix (viewport.cpp:30-35) */
@@ -463,11 +464,165 @@ struct shader {
     }
 };
 
+/* ---------------- FP64 re-shade ------------------------------------------
+ * The same shading model as `shader` (phong + bilinear + combine) with every
+ * shading quantity carried in double: vertex attributes are the FP32 v_world,
+ * normal_world and tex coords widened to double, the interpolation
+ * parameters (pl, pr, p) are the FP32 raster's, and the model's own integer
+ * steps (texel choice, 65536 fixed-point light, 8-bit truncation) are kept.
+ * Comparing its output with the FP32 colours measures the colour error of
+ * FP32 shading alone, for the same winning triangle of each pixel. */
+struct dvec3 { double x, y, z; };
+struct dvec2 { double x, y; };
+inline dvec3 D(vec3 a) { return {a.x, a.y, a.z}; }
+inline dvec3 operator+(dvec3 a, dvec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline dvec3 operator-(dvec3 a, dvec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline dvec3 operator*(dvec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline dvec2 operator+(dvec2 a, dvec2 b) { return {a.x + b.x, a.y + b.y}; }
+inline dvec2 operator-(dvec2 a, dvec2 b) { return {a.x - b.x, a.y - b.y}; }
+inline dvec2 operator*(dvec2 a, double s) { return {a.x * s, a.y * s}; }
+inline double ddot(dvec3 a, dvec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline dvec3 dnormalize(dvec3 a)
+{
+    double l = std::sqrt(ddot(a, a));
+    return l != 0 ? dvec3{a.x / l, a.y / l, a.z / l} : a;
+}
+
+struct shader64 {
+    const scene_m *sc;
+    const viewport_m *vp;
+    const shader *sh;   /* texture binding of the FP32 shader */
+    dvec3 v0, v1, v2, vleft, vright, vleftdir, vrightdir, v, vdir;
+    dvec3 n0, n1, n2, nleft, nright, nleftdir, nrightdir, n, ndir;
+    dvec2 t0, t1, t2, side_long_t_dir, side_short_t, side_short_t_dir, t_left, t_dir;
+    bool long_line_on_right;
+
+    void prepare_for_triangle(uint32_t i0, uint32_t i1, uint32_t i2, bool inverted)
+    {
+        const std::vector<mesh_vertex> &vs = sh->prim->vertices;
+        const double sgn = inverted ? -1.0 : 1.0;
+        v0 = D(vs[i0].v_world);
+        v1 = D(vs[i1].v_world);
+        v2 = D(vs[i2].v_world);
+        n0 = D(vs[i0].normal_world) * sgn;
+        n1 = D(vs[i1].normal_world) * sgn;
+        n2 = D(vs[i2].normal_world) * sgn;
+        const double tw = sh->twidth, th = sh->theight;
+        t0 = {vs[i0].tex.x * tw, vs[i0].tex.y * th};
+        t1 = {vs[i1].tex.x * tw, vs[i1].tex.y * th};
+        t2 = {vs[i2].tex.x * tw, vs[i2].tex.y * th};
+        side_long_t_dir = t2 - t0;
+    }
+    void prepare_for_upper_triangle(bool lr)
+    {
+        vleft = vright = v0;
+        nleft = nright = n0;
+        vleftdir = lr ? v1 - v0 : v2 - v0;
+        vrightdir = lr ? v2 - v0 : v1 - v0;
+        nleftdir = lr ? n1 - n0 : n2 - n0;
+        nrightdir = lr ? n2 - n0 : n1 - n0;
+        long_line_on_right = lr;
+        side_short_t = t0;
+        side_short_t_dir = t1 - t0;
+    }
+    void prepare_for_lower_triangle(bool lr)
+    {
+        if (lr) {
+            vright = v0; vrightdir = v2 - v0; vleft = v1; vleftdir = v2 - v1;
+            nright = n0; nrightdir = n2 - n0; nleft = n1; nleftdir = n2 - n1;
+        } else {
+            vleft = v0; vleftdir = v2 - v0; vright = v1; vrightdir = v2 - v1;
+            nleft = n0; nleftdir = n2 - n0; nright = n1; nrightdir = n2 - n1;
+        }
+        long_line_on_right = lr;
+        side_short_t = t1;
+        side_short_t_dir = t2 - t1;
+    }
+    void prepare_for_scanline(double pl, double pr)
+    {
+        v = vleft + vleftdir * pl;
+        vdir = vright + vrightdir * pr - v;
+        n = nleft + nleftdir * pl;
+        ndir = nright + nrightdir * pr - n;
+        if (long_line_on_right) {
+            t_left = side_short_t + side_short_t_dir * pl;
+            t_dir = t0 + side_long_t_dir * pr - t_left;
+        } else {
+            t_left = t0 + side_long_t_dir * pl;
+            t_dir = side_short_t + side_short_t_dir * pr - t_left;
+        }
+    }
+    uint32_t bilinear(dvec2 t) const
+    {
+        const uint32_t *pc = sh->tbitmap;
+        const int W = sh->twidth, H = sh->theight;
+        double v = t.x, u = t.y;   /* row coordinate comes from t.x */
+        double u1 = u - 0.5, u2 = u + 0.5, v1 = v - 0.5, v2 = v + 0.5;
+        u = std::floor(u2);
+        v = std::floor(v2);
+        int r1 = wrap_add(cvtt_d(v1), H) % H, r2 = r1 + 1 == H ? 0 : r1 + 1;
+        int c1 = wrap_add(cvtt_d(u1), W) % W, c2 = c1 + 1 == W ? 0 : c1 + 1;
+        const int64_t lim = (int64_t)W * H;
+        auto fetch = [&](int64_t r, int64_t c) {
+            int64_t idx = r * W + c;
+            idx = idx < 0 ? 0 : idx >= lim ? lim - 1 : idx;
+            return pc[idx];
+        };
+        uint32_t p11 = fetch(r1, c1), p21 = fetch(r2, c1), p12 = fetch(r1, c2), p22 = fetch(r2, c2);
+        double w11 = (u - u1) * (v - v1), w21 = (u - u1) * (v2 - v);
+        double w12 = (u2 - u) * (v - v1), w22 = (u2 - u) * (v2 - v);
+        uint32_t out = 0;
+        for (int c = 0; c < 4; c++) {
+            double s = (double)((p11 >> (8 * c)) & 0xFF) * w11 + (double)((p21 >> (8 * c)) & 0xFF) * w21 +
+                       (double)((p12 >> (8 * c)) & 0xFF) * w12 + (double)((p22 >> (8 * c)) & 0xFF) * w22;
+            out |= (uint32_t)to_u8_d(std::round(s)) << (8 * c);
+        }
+        return out;
+    }
+    int32_t phong(dvec3 P, dvec3 N) const
+    {
+        dvec3 C = dnormalize(D(vp->cam) - P);
+        double sun = -ddot(N, D(sc->sun_dir));
+        sun = sun < 0 ? 0 : sun * sc->sun_intensity;
+        double acc = 0;
+        for (const swegl_point_light &l : sc->lights) {
+            dvec3 L = P - dvec3{l.position[0], l.position[1], l.position[2]};
+            double d2 = ddot(L, L);
+            double diffuse = l.intensity / d2;
+            if (diffuse < 0.05)
+                continue;
+            L = dnormalize(L);
+            double a = -ddot(N, L);
+            if (a < 0)
+                continue;
+            diffuse *= a;
+            double s = ddot(L + N * (a * 2), C);
+            acc += s > 0 ? diffuse + std::pow(s, 32.0) * 16.0 / d2 : diffuse;
+        }
+        return cvtt_d(65536.0 * (sc->ambient + sun + acc));
+    }
+    uint32_t shade(double p) const
+    {
+        uint32_t color = bilinear(t_left + t_dir * p);
+        int32_t li = phong(v + vdir * p, dnormalize(n + ndir * p));
+        double light = li / 65536.0;
+        uint32_t out = color & 0xFF000000u;
+        for (int k = 0; k < 3; k++) {
+            double c = (color >> (8 * k)) & 0xFF;
+            uint32_t o = light < 1 ? to_u8_d(c * light)
+                                   : (uint8_t)(255 - to_u8_d((255 - c) / std::sqrt(std::sqrt(light))));
+            out |= o << (8 * k);
+        }
+        return out;
+    }
+};
+
 /* ---------------- fragment sinks ---------------------------------------- */
 /* The full renderer: z test, shade, write (renderer.cpp:486-499). */
 struct render_sink {
     viewport_m *vp;
     shader *sh;
+    shader64 *sh64 = nullptr;   /* set with viewport_m::color64 */
     uint32_t seq;
     uint64_t fragments = 0, nan_fragments = 0;
     void span(int y, int x1, int x2, interp &q)
@@ -484,6 +639,8 @@ struct render_sink {
             if (z >= vp->zb[off])
                 continue;
             *video = sh->shade(q.progress());
+            if (sh64)
+                vp->color64[off] = sh64->shade(q.progress());
             vp->zb[off] = z;
             if (vp->tid)
                 vp->tid[off] = seq;
@@ -492,10 +649,27 @@ struct render_sink {
     void prepare_for_triangle(uint32_t i0, uint32_t i1, uint32_t i2, bool inv)
     {
         sh->prepare_for_triangle(i0, i1, i2, inv);
+        if (sh64)
+            sh64->prepare_for_triangle(i0, i1, i2, inv);
     }
-    void prepare_for_upper_triangle(bool lr) { sh->prepare_for_upper_triangle(lr); }
-    void prepare_for_lower_triangle(bool lr) { sh->prepare_for_lower_triangle(lr); }
-    void prepare_for_scanline(float pl, float pr) { sh->prepare_for_scanline(pl, pr); }
+    void prepare_for_upper_triangle(bool lr)
+    {
+        sh->prepare_for_upper_triangle(lr);
+        if (sh64)
+            sh64->prepare_for_upper_triangle(lr);
+    }
+    void prepare_for_lower_triangle(bool lr)
+    {
+        sh->prepare_for_lower_triangle(lr);
+        if (sh64)
+            sh64->prepare_for_lower_triangle(lr);
+    }
+    void prepare_for_scanline(float pl, float pr)
+    {
+        sh->prepare_for_scanline(pl, pr);
+        if (sh64)
+            sh64->prepare_for_scanline(pl, pr);
+    }
 };
 
 /* Coverage-only sink for the single-triangle probe. */
@@ -920,14 +1094,22 @@ void render_viewport(scene_m &sc, viewport_m &vp, const oracle_viewport &ov, ora
         vp.zb[i] = mz;
         if (vp.tid)
             vp.tid[i] = 0xFFFFFFFFu;
+        if (vp.color64)
+            vp.color64[i] = 0;
     }
 
     shader sh{};
     sh.sc = &sc;
     sh.vp = &vp;
+    shader64 sh64{};
+    sh64.sc = &sc;
+    sh64.vp = &vp;
+    sh64.sh = &sh;
     render_sink sink;
     sink.vp = &vp;
     sink.sh = &sh;
+    if (vp.color64)
+        sink.sh64 = &sh64;
     draw_ctx dc;
     dc.vp = &vp;
     dc.sink = &sink;
@@ -1030,6 +1212,7 @@ int oracle_render(const swegl_scene_desc *scene, oracle_viewport *vps, uint32_t 
         vp.pitch_words = o.pitch / 4;
         vp.zb = o.zbuffer;
         vp.tid = o.triangle_id;
+        vp.color64 = o.color64;
         vp.view = o.view;
         vp.proj = o.projection;
         vp.cam = {o.camera_position[0], o.camera_position[1], o.camera_position[2]};

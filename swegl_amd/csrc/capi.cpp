@@ -492,6 +492,15 @@ extern "C" int swegl_target_read_rect(swegl_ctx *c, swegl_target *t, int32_t x, 
     return SWEGL_OK;
 }
 
+extern "C" int swegl_target_copy(swegl_ctx *c, swegl_target *dst, swegl_target *src)
+{
+    if (!c || !dst || !src || dst->w != src->w || dst->h != src->h || dst->kind != src->kind)
+        return fail(SWEGL_EINVAL, "targets differ");
+    HIPCHK(hipMemcpyAsync(dst->buf.p, src->buf.p, (size_t)src->w * src->h * 4, hipMemcpyDeviceToDevice,
+                          c->stream));
+    return SWEGL_OK;
+}
+
 extern "C" int swegl_target_write(swegl_ctx *c, swegl_target *t, const void *src, int32_t pitch)
 {
     if (!c || !t || !src || pitch < t->w * 4)
@@ -828,6 +837,7 @@ static void m34_from(const float *m16, m34 &o) { memcpy(o.m, m16, 12 * sizeof(fl
 struct Wave {
     std::vector<JobDev> jd;
     uint32_t vblk = 0, tblk = 0, dblk = 0, tiles = 0, n_ctr = 0;
+    int rmax = 0;             // largest DoF radius (sizes the DoF LDS)
     uint64_t cap = 0;
     size_t job_offset = 0;   // byte offset of this wave's table in Resources::jobs
 };
@@ -917,8 +927,10 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         J.tblock0 = W.tblk;
         W.tblk += (s->n_tris + 255) / 256;
         J.dblock0 = W.dblk;
-        if (dof)
-            W.dblk += (uint32_t)(((v.w + 15) / 16) * ((v.h + 15) / 16));
+        if (dof) {
+            W.dblk += dof_blocks(v.w, v.h);
+            W.rmax = std::max(W.rmax, v.dof_max_radius);
+        }
         tris_total += s->n_tris;
     }
     W.n_ctr = SW_CTR_BIG0 + nj;
@@ -982,7 +994,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     STAGE("scan", launch_scan(B, st));
     STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
     STAGE("raster", launch_raster(B, c->tw, c->th, groups, st));
-    STAGE("dof", launch_dof(B, W.dblk, st));
+    STAGE("dof", launch_dof(B, W.dblk, W.rmax, st));
 #undef STAGE
     return SWEGL_OK;
 }
@@ -1067,7 +1079,8 @@ static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
                     "/" + std::to_string(c->groups_per_cu);
     for (const Wave &W : waves) {
         k += "|" + std::to_string(W.jd.size()) + ":" + std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
-             "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap);
+             "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap) + "," +
+             std::to_string(W.rmax);
     }
     return k;
 }

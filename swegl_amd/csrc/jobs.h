@@ -44,6 +44,13 @@ struct TexDev {
 };
 
 #define SW_MAX_DOF_RADIUS 16
+// Depth-of-field output tile (pixels) of one 256-thread workgroup.
+#define SW_DOF_TW 64
+#define SW_DOF_TH 16
+__host__ __device__ inline uint32_t dof_blocks(int w, int h)
+{
+    return (uint32_t)(((w + SW_DOF_TW - 1) / SW_DOF_TW) * ((h + SW_DOF_TH - 1) / SW_DOF_TH));
+}
 
 struct JobDev {
     // ---- resident scene

@@ -718,7 +718,7 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
 
 // ---------------------------------------------------------------- k_dof
 // Depth of field (build spec of post_shader_depth_box, post_shaders.hpp:51-131
-// with remap_clipped, lerp.hpp:24-43): 16x16 output pixels per block, colour
+// with remap_clipped, lerp.hpp:24-43): 64x16 output pixels per block, colour
 // and blur factor of the tile plus an R_max halo staged in LDS.
 __device__ __forceinline__ float blur_factor(float z, float fdist, float fdepth, float rmax)
 {
@@ -741,69 +741,112 @@ __device__ __forceinline__ float blur_factor(float z, float fdist, float fdepth,
     return 0.0f + rmax * x;
 }
 
-#define DOF_T 16
-#define DOF_S (DOF_T + 2 * SW_MAX_DOF_RADIUS)
+// LDS bytes of one DoF workgroup for radius R: colour + blur factor of the
+// output tile plus an R halo on every side.
+__host__ __device__ inline size_t dof_lds_bytes(int R)
+{
+    return (size_t)(SW_DOF_TW + 2 * R) * (SW_DOF_TH + 2 * R) * 8;
+}
+
+// 256 threads as 64 columns x 4 rows.  Consecutive workgroups run on
+// different XCDs (round robin over 8), so the job's tiles are dealt out in
+// 8 contiguous runs, one per XCD: neighbouring tiles, whose halos overlap,
+// then share that XCD's L2.
 __global__ __launch_bounds__(256) void k_dof(BatchDev B)
 {
-    __shared__ uint32_t s_col[DOF_S * DOF_S];
-    __shared__ float s_bf[DOF_S * DOF_S];
+    extern __shared__ uint32_t s_dyn[];
     const JobDev &J = B.jobs[find_job<JD>(B, blockIdx.x)];
-    const int lb = (int)(blockIdx.x - J.dblock0);
-    const int bx = (J.V.vw + DOF_T - 1) / DOF_T;
-    const int ox = (lb % bx) * DOF_T, oy = (lb / bx) * DOF_T;   // viewport-relative
+    const uint32_t n = dof_blocks(J.V.vw, J.V.vh), q = n / 8;
+    uint32_t lb = blockIdx.x - J.dblock0;
+    if (lb < 8 * q)
+        lb = (lb % 8) * q + lb / 8;
+    const int bx = (J.V.vw + SW_DOF_TW - 1) / SW_DOF_TW;
+    const int ox = (int)(lb % bx) * SW_DOF_TW, oy = (int)(lb / bx) * SW_DOF_TH;   // viewport-relative
     const int R = J.rmax;
-    const int S = DOF_T + 2 * R;
+    const int SW = SW_DOF_TW + 2 * R, SH = SW_DOF_TH + 2 * R;
+    uint32_t *s_col = s_dyn;
+    float *s_bf = (float *)(s_dyn + SW * SH);
     const float rmax = (float)R;
-    for (int i = threadIdx.x; i < S * S; i += 256) {
-        int lx = ox - R + i % S, ly = oy - R + i / S;
-        if (lx >= 0 && ly >= 0 && lx < J.V.vw && ly < J.V.vh) {
-            s_col[i] = J.out_color[(int64_t)(ly + J.V.vy - J.out_oy) * J.out_pitch + (lx + J.V.vx - J.out_ox)];
-            s_bf[i] = blur_factor(J.zbuf[(int64_t)ly * J.V.vw + lx], J.fdist, J.fdepth, rmax);
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // Every load of the halo is issued before the first is used, so the
+    // workgroup waits for memory once rather than once per row.
+    constexpr int ROWS = (SW_DOF_TH + 2 * SW_MAX_DOF_RADIUS + 3) / 4;
+    constexpr int COLS = (SW_DOF_TW + 2 * SW_MAX_DOF_RADIUS + 63) / 64;
+    uint32_t cv[ROWS][COLS];
+    float zv[ROWS][COLS];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+        const int r = ty + 4 * k, ly = oy - R + r;
+        const bool row_ok = r < SH && ly >= 0 && ly < J.V.vh;
+        const uint32_t *crow = J.out_color + (int64_t)(ly + J.V.vy - J.out_oy) * J.out_pitch + (J.V.vx - J.out_ox);
+        const float *zrow = J.zbuf + (int64_t)ly * J.V.vw;
+#pragma unroll
+        for (int m = 0; m < COLS; m++) {
+            const int c = tx + 64 * m, lx = ox - R + c;
+            if (row_ok && c < SW && lx >= 0 && lx < J.V.vw) {
+                cv[k][m] = crow[lx];
+                zv[k][m] = zrow[lx];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+        const int r = ty + 4 * k, ly = oy - R + r;
+        const bool row_ok = r < SH && ly >= 0 && ly < J.V.vh;
+#pragma unroll
+        for (int m = 0; m < COLS; m++) {
+            const int c = tx + 64 * m, lx = ox - R + c;
+            if (row_ok && c < SW && lx >= 0 && lx < J.V.vw) {
+                s_col[r * SW + c] = cv[k][m];
+                s_bf[r * SW + c] = blur_factor(zv[k][m], J.fdist, J.fdepth, rmax);
+            }
         }
     }
     __syncthreads();
-    const int px = threadIdx.x % DOF_T, py = threadIdx.x / DOF_T;
-    const int x = ox + px, y = oy + py;
-    if (x >= J.V.vw || y >= J.V.vh)
-        return;
-    const int c = (py + R) * S + (px + R);
-    const uint32_t src = s_col[c];
-    const int radius = min(max(cvtt(s_bf[c]), 0), R);   // bf is in [0, R] for finite inputs
-    uint32_t out = src;
-    if (radius != 0) {
-        int b = 0, g = 0, r = 0, count = 0;
-        auto tap = [&](int i, int j) {
-            int k = (j - oy + R) * S + (i - ox + R);
-            if (s_bf[k] != 0) {
-                count++;
-                uint32_t p = s_col[k];
-                b += p & 0xFF;
-                g += (p >> 8) & 0xFF;
-                r += (p >> 16) & 0xFF;
-            }
-        };
-        if (J.post == 1) {   // box [x-r, x+r) x [y-r, y+r)
-            for (int j = max(0, y - radius); j < min(J.V.vh, y + radius); j++)
-                for (int i = max(0, x - radius); i < min(J.V.vw, x + radius); i++)
-                    tap(i, j);
-        } else {             // 3x3 taps at {-r, 0, r}
-            for (int a = -1; a <= 1; a++) {
-                int j = y + a * radius;
-                if (j < 0 || j >= J.V.vh)
-                    continue;
-                for (int e = -1; e <= 1; e++) {
-                    int i = x + e * radius;
-                    if (i < 0 || i >= J.V.vw)
+    for (int py = ty; py < SW_DOF_TH; py += 4) {
+        const int px = tx;
+        const int x = ox + px, y = oy + py;
+        if (x >= J.V.vw || y >= J.V.vh)
+            continue;
+        const int c = (py + R) * SW + (px + R);
+        const uint32_t src = s_col[c];
+        const int radius = min(max(cvtt(s_bf[c]), 0), R);   // bf is in [0, R] for finite inputs
+        uint32_t out = src;
+        if (radius != 0) {
+            int b = 0, g = 0, r = 0, count = 0;
+            auto tap = [&](int i, int j) {
+                const int k = (j - oy + R) * SW + (i - ox + R);
+                if (s_bf[k] != 0) {
+                    count++;
+                    const uint32_t q = s_col[k];
+                    b += q & 0xFF;
+                    g += (q >> 8) & 0xFF;
+                    r += (q >> 16) & 0xFF;
+                }
+            };
+            if (J.post == 1) {   // box [x-r, x+r) x [y-r, y+r)
+                for (int j = max(0, y - radius); j < min(J.V.vh, y + radius); j++)
+                    for (int i = max(0, x - radius); i < min(J.V.vw, x + radius); i++)
+                        tap(i, j);
+            } else {             // 3x3 taps at {-r, 0, r}
+                for (int a = -1; a <= 1; a++) {
+                    const int j = y + a * radius;
+                    if (j < 0 || j >= J.V.vh)
                         continue;
-                    tap(i, j);
+                    for (int e = -1; e <= 1; e++) {
+                        const int i = x + e * radius;
+                        if (i < 0 || i >= J.V.vw)
+                            continue;
+                        tap(i, j);
+                    }
                 }
             }
+            if (count)
+                out = (uint32_t)(uint8_t)(b / count) | ((uint32_t)(uint8_t)(g / count) << 8) |
+                      ((uint32_t)(uint8_t)(r / count) << 16) | (255u << 24);
         }
-        if (count)
-            out = (uint32_t)(uint8_t)(b / count) | ((uint32_t)(uint8_t)(g / count) << 8) |
-                  ((uint32_t)(uint8_t)(r / count) << 16) | (255u << 24);
+        J.surface[(int64_t)(y + J.V.vy) * J.surface_pitch + (x + J.V.vx)] = out;
     }
-    J.surface[(int64_t)(y + J.V.vy) * J.surface_pitch + (x + J.V.vx)] = out;
 }
 
 __global__ void k_fill32(uint32_t *p, uint64_t n, uint32_t w)
@@ -886,10 +929,10 @@ hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups,
     return hipSuccess;
 }
 
-hipError_t launch_dof(const BatchDev &B, uint32_t blocks, hipStream_t s)
+hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStream_t s)
 {
     if (blocks)
-        LAUNCH_CHECK(k_dof, dim3(blocks), dim3(256), 0, s, B);
+        LAUNCH_CHECK(k_dof, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B);
     return hipSuccess;
 }
 

@@ -48,6 +48,7 @@ def lib():
             "swegl_target_read": (C.c_int, [vp, vp, vp, C.c_int32]),
             "swegl_target_write": (C.c_int, [vp, vp, vp, C.c_int32]),
             "swegl_target_fill": (C.c_int, [vp, vp, C.c_uint32]),
+            "swegl_target_copy": (C.c_int, [vp, vp, vp]),
             "swegl_scene_upload": (C.c_int, [vp, C.POINTER(A.SceneDesc), C.POINTER(vp)]),
             "swegl_scene_update_pose": (C.c_int, [vp, vp, C.POINTER(A.SceneDesc)]),
             "swegl_scene_destroy": (C.c_int, [vp, vp]),

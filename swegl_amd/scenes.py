@@ -213,3 +213,116 @@ def batch_scenes(seed=0, count=256, tmin=1_000, tmax=200_000,
                                    dof=dof)
         out.append((s, vps))
     return out
+
+
+# ------------------------------------------------------------------ heavy
+def _project(cam, p):
+    """(x', y', z') of world points p (..., 3): view, then projection rows
+    0-2 (vertex_shaders.hpp:35-71 before the divide by |z'|)."""
+    V = cam.m_viewmatrix.astype(np.float64)
+    P = cam.m_projectionmatrix.astype(np.float64)
+    h = np.concatenate([p, np.ones(p.shape[:-1] + (1,))], -1)
+    c = h @ V.T
+    c[..., 3] = 1.0
+    return (c @ P.T)[..., :3]
+
+
+def cluster_mesh(rng, cam, width, n_clusters=500, quads=(25, 20), depth=(-2.0, 1.0),
+                 edge_px=(0.5, 4.0), stack=(8, 20), tex_side=4096):
+    """`n_clusters` patches of quads[0] x quads[1] cells (2 triangles each),
+    facing the camera, piled in stacks of stack[0]..stack[1] patches at
+    random depths over one screen point each, so covered pixels see that
+    many layers.  Each patch's cell edge projects to edge_px[0]..edge_px[1]
+    pixels; its texture coordinates cover about 1.5 texels per pixel of a
+    tex_side^2 texture.  Returns (positions, normals, uvs, indices)."""
+    qx, qy = quads
+    nvx, nvy = qx + 1, qy + 1
+    sizes = []
+    while sum(sizes) < n_clusters:
+        sizes.append(int(rng.integers(stack[0], stack[1] + 1)))
+    sizes[-1] -= sum(sizes) - n_clusters
+    centres = rng.uniform(-0.8, 0.8, (len(sizes), 2))
+    pos, nor, uv, idx = [], [], [], []
+    gx, gy = np.meshgrid(np.arange(nvx, dtype=np.float64) - qx / 2, np.arange(nvy, dtype=np.float64) - qy / 2,
+                         indexing="ij")
+    i = np.arange(qx)
+    I, J = np.meshgrid(i, np.arange(qy), indexing="ij")
+    a = (I * nvy + J).reshape(-1)
+    b = ((I + 1) * nvy + J).reshape(-1)
+    c = ((I + 1) * nvy + J + 1).reshape(-1)
+    d = (I * nvy + J + 1).reshape(-1)
+    cell_tris = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)], 1).reshape(-1)
+    base = 0
+    for s_i, n_s in enumerate(sizes):
+        for _ in range(n_s):
+            zw = rng.uniform(*depth)
+            sx, sy = centres[s_i] + rng.uniform(-0.01, 0.01, 2)
+            # world x, y landing on (sx, sy): x', y' are affine in x, y at fixed depth
+            p0 = _project(cam, np.array([0.0, 0.0, zw]))
+            px = _project(cam, np.array([1.0, 0.0, zw]))
+            py = _project(cam, np.array([0.0, 1.0, zw]))
+            zabs = abs(p0[2])
+            xw = (sx * zabs - p0[0]) / (px[0] - p0[0])
+            yw = (sy * zabs - p0[1]) / (py[1] - p0[1])
+            e = rng.uniform(*edge_px)
+            cell = e * 2.0 * zabs / (width * (px[0] - p0[0]))   # world size of an e-pixel edge
+            bump = rng.uniform(-0.3, 0.3, gx.shape) * cell
+            P = np.stack([xw + gx * cell, yw + gy * cell, zw + bump], -1).reshape(-1, 3)
+            tilt = rng.uniform(-0.4, 0.4, 2)
+            N = np.stack([np.full(gx.shape, tilt[0]), np.full(gx.shape, tilt[1]), np.ones(gx.shape)], -1)
+            N = N + rng.uniform(-0.15, 0.15, N.shape)
+            N /= np.sqrt((N * N).sum(-1, keepdims=True))
+            span = 1.5 * e * np.array([qx, qy]) / tex_side
+            off = rng.uniform(0.0, 1.0, 2)
+            T = np.stack([off[0] + (gx + qx / 2) / qx * span[0], off[1] + (gy + qy / 2) / qy * span[1]], -1)
+            pos.append(P.reshape(-1, 3))
+            nor.append(N.reshape(-1, 3))
+            uv.append(T.reshape(-1, 2))
+            idx.append(cell_tris + base)
+            base += nvx * nvy
+    return (np.concatenate(pos).astype(np.float32), np.concatenate(nor).astype(np.float32),
+            np.concatenate(uv).astype(np.float32), np.concatenate(idx).astype(np.uint32))
+
+
+def heavy_scene(seed=0, n_clusters=500, n_viewports=4, width=1920, height=1080, tex=4096, n_lights=8,
+                dof=True, dof_radius=8):
+    """Workload 'heavy' (one scene): n_clusters x 1000 triangles in stacked
+    patches (500 clusters = 500,000 triangles), a tex^2 texture, sun +
+    n_lights point lights, n_viewports viewports of width x height onto
+    their own surfaces, from cameras a little apart, 9-tap depth of field."""
+    rng = np.random.default_rng(seed)
+    s = scene_t()
+    s.images = [random_texture(rng, tex, tex)]
+    s.materials = [material_t(pixel_colors(255, 255, 255, 255), 1, 1, 0, False)]
+    vps = []
+    for k in range(n_viewports):
+        surface = Surface(width, height)
+        vp = viewport_t(0, 0, width, height, surface, pixel_shader_light_and_texture(), 0)
+        cam = vp.m_camera
+        cam.translate(float(f32(0.12 * (k - (n_viewports - 1) / 2.0))), 0.0, -3.0)
+        set_fov(cam, f32(1.0 * width / height))
+        vps.append(vp)
+    pos, nor, uv, idx = cluster_mesh(rng, vps[0].m_camera, width, n_clusters=n_clusters, tex_side=tex)
+    s.nodes = [node_t([primitive_t(pos, nor, uv, idx, A.TRIANGLES, 0)])]
+    s.root_nodes = [0]
+    s.ambient_light_intensity = float(f32(0.1))
+    s.sun_direction = seeded_sun(rng)
+    s.sun_intensity = float(f32(0.5))
+    for _ in range(n_lights):
+        p = np.array([rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.5, 2.5)], np.float32)
+        s.point_source_lights.append(point_source_light(p, f32(rng.uniform(0.5, 2.0))))
+    # focus on the middle of the depth range (z' of a point at world z = -0.5)
+    zmid = float(abs(_project(vps[0].m_camera, np.array([0.0, 0.0, -0.5]))[2]))
+    for vp in vps:
+        if dof:
+            vp.set_post_shader(post_shader_depth_box(zmid, 2.0, vp, max_radius=dof_radius, taps="tap9"))
+        else:
+            vp.set_post_shader(post_shader_t())
+    return s, vps
+
+
+def heavy_scenes(seed=0, count=64, **kw):
+    """Workload 'heavy': `count` heavy scenes (defaults: 500k triangles,
+    4 x 1920x1080 viewports, 4096^2 texture, sun + 8 point lights, DoF)."""
+    rng = np.random.default_rng(seed)
+    return [heavy_scene(int(rng.integers(0, 1 << 31)), **kw) for _ in range(count)]

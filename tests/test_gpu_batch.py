@@ -38,6 +38,22 @@ def test_batch_equals_plain_and_oracle(ctx, graph):
             compare(got[k][0], want, colour_tol=2)
 
 
+def test_heavy_batch_equals_plain(ctx):
+    """Multi-viewport jobs (4 viewports per scene) batched in one graph equal
+    each scene rendered alone."""
+    pairs = scenes.heavy_scenes(4, count=3, n_clusters=30, width=320, height=180, tex=256)
+    b = Batch(ctx, pairs)
+    b.render(graph=True)
+    got = b.frames()
+    for k, (s, vps) in enumerate(pairs):
+        alone = gpu_render(ctx, s, vps)
+        for i in range(len(vps)):
+            assert same_frame(got[k][i], alone[i]), (k, i)
+    want = oracle.render(*pairs[0])
+    for i in range(len(pairs[0][1])):
+        compare(got[0][i], want[i], colour_tol=2)
+
+
 def test_graph_replays_follow_camera_and_stay_exact(ctx):
     """Capture once, replay with new camera poses and eager work in between;
     every replay equals the eager render of the same pose."""

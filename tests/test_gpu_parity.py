@@ -53,6 +53,29 @@ def test_heightfield_1m_1080p(ctx):
     print(f"heightfield 1M: fragments {got.stats['fragments']}, colour max diff {mx} at {nd} px")
 
 
+@pytest.mark.parametrize("seed,clusters,w,h,tex", [(1, 40, 480, 270, 512), (2, 120, 640, 360, 1024)])
+def test_heavy_small(ctx, seed, clusters, w, h, tex):
+    """Stacked 1000-triangle clusters, 4 viewports from different cameras,
+    sun + 8 point lights, 9-tap DoF."""
+    scene, vps = scenes.heavy_scene(seed, n_clusters=clusters, width=w, height=h, tex=tex)
+    want = oracle.render(scene, vps)
+    got = gpu_render(ctx, scene, vps)
+    for g, wf in zip(got, want):
+        compare(g, wf, colour_tol=2)
+    assert got[0].stats["fragments"] == sum(f.stats["fragments"] for f in want)
+
+
+def test_heavy_full_scene(ctx):
+    """One full heavy scene: 500,000 triangles, 4 x 1920x1080, 4096^2 texture."""
+    scene, vps = scenes.heavy_scene(5)
+    want = oracle.render(scene, vps)
+    got = gpu_render(ctx, scene, vps)
+    for g, wf in zip(got, want):
+        compare(g, wf, colour_tol=2)
+    assert got[0].stats["fragments"] == sum(f.stats["fragments"] for f in want)
+    assert got[0].stats["drawn_subtriangles"] == sum(f.stats["drawn_subtriangles"] for f in want)
+
+
 @pytest.mark.parametrize("limit", [0, 1, 4])
 def test_bin_limit_paths_agree(ctx, limit):
     """Triangles over the bin limit take the per-viewport list that every
