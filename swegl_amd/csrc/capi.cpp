@@ -963,7 +963,8 @@ static int size_resources(Resources &R, std::vector<Wave> &waves)
 }
 
 // Enqueue the kernels of one wave (no allocation: safe under graph capture).
-static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool first_wave, bool capturing)
+static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool first_wave, bool capturing,
+                       bool want_stats)
 {
     BatchDev B;
     B.jobs = (const JobDev *)((char *)R.jobs.p + W.job_offset);
@@ -976,6 +977,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     B.bin_capacity = (uint32_t)W.cap;
     B.ctr = R.ctr.as<uint32_t>();
     B.big_tiles = c->big_tiles;
+    B.stats = want_stats ? 1u : 0u;
     // statistics are zeroed once per render by the first wave; control words
     // and big-list lengths by every wave
     const uint32_t zero0 = first_wave ? 0 : SW_CTR_CONTROL;
@@ -987,8 +989,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
         if (e_ != hipSuccess)                                                          \
             return fail(SWEGL_EIO, "%s launch: %s", name, hipGetErrorString(e_));       \
     } while (0)
-    STAGE("clear", launch_clear(B, zero0, W.n_ctr - zero0, st));
-    STAGE("vertex", launch_vertex(B, W.vblk, st));
+    STAGE("vertex", launch_vertex(B, W.vblk, zero0, W.n_ctr - zero0, st));
     STAGE("mark", launch_mark(B, W.tblk, st));
     STAGE("bin_count", launch_bin_count(B, W.tblk, st));
     STAGE("scan", launch_scan(B, st));
@@ -1065,7 +1066,7 @@ static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swe
     for (size_t w = 0; w < waves.size(); w++) {
         if ((rc = upload(L, (char *)L.res.jobs.p + waves[w].job_offset, waves[w].jd.data(),
                          waves[w].jd.size() * sizeof(JobDev))) ||
-            (rc = launch_wave(c, L, L.res, waves[w], w == 0, false)))
+            (rc = launch_wave(c, L, L.res, waves[w], w == 0, false, stats != nullptr)))
             return rc;
     }
     return stats ? read_stats(c, L, L.res, stats) : SWEGL_OK;
@@ -1091,7 +1092,7 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
     int rc = prepare(c, c->lane0, all, waves);
     if (rc)
         return rc;
-    const std::string key = graph_key(c, waves);
+    const std::string key = graph_key(c, waves) + (stats ? "/stats" : "");
     auto it = c->graphs.find(key);
     GraphEntry *G;
     if (it == c->graphs.end()) {
@@ -1110,7 +1111,7 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
                                       c->stream);
         int lrc = e == hipSuccess ? SWEGL_OK : fail(SWEGL_EIO, "capture copy: %s", hipGetErrorString(e));
         for (size_t w = 0; lrc == SWEGL_OK && w < waves.size(); w++)
-            lrc = launch_wave(c, c->lane0, g->res, waves[w], w == 0, true);
+            lrc = launch_wave(c, c->lane0, g->res, waves[w], w == 0, true, stats != nullptr);
         hipGraph_t graph = nullptr;
         hipError_t ee = hipStreamEndCapture(c->stream, &graph);
         if (lrc)

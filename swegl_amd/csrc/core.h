@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #define SW_HD __host__ __device__ __forceinline__
+#define SW_D __device__ __forceinline__
 
 namespace sw {
 
@@ -40,8 +41,34 @@ SW_HD f2 operator*(f2 a, float s) { return {a.x * s, a.y * s}; }
 SW_HD f3 neg(f3 a) { return {-a.x, -a.y, -a.z}; }
 // vector_t::dot (points.hpp:93): (xx + yy) + zz
 SW_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Correctly rounded a / b for several numerators over one denominator b,
+// for callers that know 1 <= b <= 2^20 and a == 0 or 2^-60 <= |a| <= 2^60.
+// The compiler expands each IEEE '/' into: v_div_scale of a and b, r =
+// rcp(b) refined by one fma step, q = a*r, two fma residual corrections
+// (the last through v_div_fmas), then v_div_fixup.  In that range the
+// scaling and the fixup are identities, so the refined reciprocal depends
+// on b alone and is computed once; each quotient then takes five
+// instructions and equals '/' bit for bit (a zero numerator keeps its
+// signed zero a*r).
+struct FDiv {
+    float b, r;
+    SW_D explicit FDiv(float den) : b(den)
+    {
+        const float r0 = __builtin_amdgcn_rcpf(den);
+        r = __builtin_fmaf(__builtin_fmaf(-den, r0, 1.0f), r0, r0);
+    }
+    SW_D float operator()(float a) const
+    {
+        const float q0 = a * r;
+        float q = __builtin_fmaf(__builtin_fmaf(-b, q0, a), r, q0);
+        q = __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+        return a == 0.0f ? q0 : q;
+    }
+};
+
 // vector_t::normalize (points.hpp:80-90)
-SW_HD f3 normalize(f3 a)
+SW_D f3 normalize(f3 a)
 {
     float l = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
     if (l != 0) {
@@ -53,7 +80,7 @@ SW_HD f3 normalize(f3 a)
 }
 // cross() builds a normal_t, which normalises (points.cpp:32-37).  Only the
 // z component is ever read on the path.
-SW_HD float cross_nz(f3 l, f3 r)
+SW_D float cross_nz(f3 l, f3 r)
 {
     return normalize({l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x}).z;
 }
@@ -70,7 +97,7 @@ SW_HD f3 xform(const m34 &M, f3 v)
 }
 // rotate(n, m) then .normalize() then normal_t::operator= : three
 // normalisations (vertex_shaders.hpp:63, points.cpp:44-49, points.hpp:143-150)
-SW_HD f3 normal_world(const m34 &RS, f3 n)
+SW_D f3 normal_world(const m34 &RS, f3 n)
 {
     const float *m = RS.m;
     f3 r = {m[0] * n.x + m[1] * n.y + m[2] * n.z, m[4] * n.x + m[5] * n.y + m[6] * n.z,
@@ -87,7 +114,7 @@ struct ViewXf {
 };
 
 // world -> frustum coordinates (vertex_shaders.hpp:35-47, 61-71)
-SW_HD f3 world_to_frustum(const ViewXf &V, f3 w)
+SW_D f3 world_to_frustum(const ViewXf &V, f3 w)
 {
     f3 c = xform(V.view, w);
     f3 p = xform(V.proj, c);
@@ -195,7 +222,7 @@ struct Setup {
 
 // fill_triangle_2 setup (renderer.cpp:361-460).  Returns false when the
 // sub-triangle spans no scanline (y0 == y2).
-SW_HD bool setup_subtri(const SubTri &t, const ViewXf &V, Setup &S)
+SW_D bool setup_subtri(const SubTri &t, const ViewXf &V, Setup &S)
 {
     SVert a = t.v[0], b = t.v[1], c = t.v[2];
     S.inverted = !t.front;
@@ -289,7 +316,7 @@ struct Span {
     Interp q;       // displaced to x1
     float pl, pr;   // side progresses passed to prepare_for_scanline
 };
-SW_HD bool span_of(const Row &R, const ViewXf &V, Span &sp)
+SW_D bool span_of(const Row &R, const ViewXf &V, Span &sp)
 {
     int x1 = cvtt(ceilf(R.left.x));
     int x2 = cvtt(ceilf(R.right.x));

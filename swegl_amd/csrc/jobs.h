@@ -107,6 +107,7 @@ struct BatchDev {
     uint32_t bin_capacity;
     uint32_t big_tiles;       // per-triangle bin limit; larger ones go to the big list
     uint32_t *ctr;            // counter words, layout below
+    uint32_t stats;           // 1: accumulate the statistics counters
 };
 
 // Counter block: [0, 1024) statistics as 64 slots x 8 u64 (slot = block

@@ -111,16 +111,6 @@ __device__ __forceinline__ void stat_add(const BatchDev &B, int idx, unsigned lo
     atomicAdd(&st[(blockIdx.x & (SW_STAT_SLOTS - 1)) * 8 + idx], v);
 }
 
-// ------------------------------------------------------------- k_clear
-__global__ void k_clear(BatchDev B, uint32_t w0, uint32_t n_words)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < B.n_tiles)
-        B.tile_count[i] = 0;
-    if (i < n_words)
-        B.ctr[w0 + i] = 0;
-}
-
 // ------------------------------------------------------------- k_world
 // original_to_world (vertex_shaders.hpp:16-33) and normal_world
 // (:61-63).  Both depend only on the scene and its node poses, so they are
@@ -148,8 +138,15 @@ __global__ __launch_bounds__(256) void k_world(WorldArgs W)
 // ------------------------------------------------------------ k_vertex
 // world_to_camera_or_frustum (vertex_shaders.hpp:35-52, 61-71) per viewport:
 // frustum coordinates and cleared visibility marks.
-__global__ __launch_bounds__(256) void k_vertex(BatchDev B)
+// It also zeroes the tile counts and counter words [w0, w0 + n_words) that
+// the binning kernels after it accumulate into.
+__global__ __launch_bounds__(256) void k_vertex(BatchDev B, uint32_t w0, uint32_t n_words)
 {
+    const uint32_t gi = blockIdx.x * 256u + threadIdx.x, stride = gridDim.x * 256u;
+    for (uint32_t k = gi; k < B.n_tiles; k += stride)
+        B.tile_count[k] = 0;
+    for (uint32_t k = gi; k < n_words; k += stride)
+        B.ctr[w0 + k] = 0;
     const JobDev &J = B.jobs[find_job<JV>(B, blockIdx.x)];
     const uint32_t i = (blockIdx.x - J.vblock0) * 256u + threadIdx.x;
     if (i >= J.n_vertices)
@@ -400,6 +397,8 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
         }
         wave_add<false>(B.tile_count, key, act);
     }
+    if (!B.stats)
+        return;
     __shared__ uint32_t s_acc[3];
     if (threadIdx.x < 3)
         s_acc[threadIdx.x] = 0;
@@ -677,12 +676,14 @@ __device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *k
         if (J.tid)
             J.tid[zo] = seq;
     }
-    if (frags)
-        atomicAdd(&s_cnt[0], frags);
-    if (nans)
-        atomicAdd(&s_cnt[1], nans);
-    if (oob)
-        atomicAdd(&s_cnt[2], oob);
+    if (B.stats) {
+        if (frags)
+            atomicAdd(&s_cnt[0], frags);
+        if (nans)
+            atomicAdd(&s_cnt[1], nans);
+        if (oob)
+            atomicAdd(&s_cnt[2], oob);
+    }
 }
 
 // Persistent: each workgroup takes tiles from the batch's work queue until
@@ -695,7 +696,7 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
     __shared__ uint32_t s_tile;
     if (threadIdx.x < 3)
         s_cnt[threadIdx.x] = 0;
-    // first tile by block index, the rest from the queue
+    // first tile by block index, the rest from the queue as workgroups free up
     uint32_t g = blockIdx.x;
     while (g < B.n_tiles) {
         raster_tile<TW, TH>(B, g, keys, s_cnt);
@@ -706,7 +707,7 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
         g = s_tile;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (B.stats && threadIdx.x == 0) {
         if (s_cnt[0])
             stat_add(B, SW_ST_FRAGMENTS, s_cnt[0]);
         if (s_cnt[1])
@@ -720,7 +721,11 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
 // Depth of field (build spec of post_shader_depth_box, post_shaders.hpp:51-131
 // with remap_clipped, lerp.hpp:24-43): 64x16 output pixels per block, colour
 // and blur factor of the tile plus an R_max halo staged in LDS.
-__device__ __forceinline__ float blur_factor(float z, float fdist, float fdepth, float rmax)
+// `by` divides by fdepth - 1 (the same for the whole job); `fast` when
+// 1 <= fdepth - 1 <= 2^20, where FDiv's quotient equals '/' (the numerator
+// t - 1 lies in [2^-23, fdepth - 1] on that branch).
+__device__ __forceinline__ float blur_factor(float z, float fdist, float fdepth, float rmax, const FDiv &by,
+                                             bool fast)
 {
     float t = fabsf(fdist - z);
     // inv_lerp_clipped(1, fdepth, t)
@@ -732,13 +737,32 @@ __device__ __forceinline__ float blur_factor(float z, float fdist, float fdepth,
     else if (t >= fdepth)
         x = 1.0f;
     else
-        x = (t - 1.0f) / (fdepth - 1.0f);
+        x = fast ? by(t - 1.0f) : (t - 1.0f) / (fdepth - 1.0f);
     // lerp_clipped(0, rmax, x)
     if (x <= 0)
         return 0.0f;
     if (x >= rmax)
         return rmax;
     return 0.0f + rmax * x;
+}
+
+// Exact floor(x / c) for 1 <= c <= 1024 and x < 2^18 (a tap count and a
+// sum of 8-bit channels) as a high multiply by m = ceil(2^32 / c): the
+// error x * (m * c - 2^32) / 2^32 < 2^18 * 2^10 / 2^32 stays below 1/c.
+struct DofRecip {
+    uint32_t m[4 * SW_MAX_DOF_RADIUS * SW_MAX_DOF_RADIUS + 1];
+};
+constexpr DofRecip make_dof_recip()
+{
+    DofRecip t{};
+    for (uint32_t c = 2; c <= 4u * SW_MAX_DOF_RADIUS * SW_MAX_DOF_RADIUS; c++)
+        t.m[c] = 0xFFFFFFFFu / c + 1u;
+    return t;
+}
+__constant__ DofRecip k_dof_recip = make_dof_recip();
+__device__ __forceinline__ uint32_t div_count(uint32_t x, uint32_t c)
+{
+    return c == 1 ? x : __umulhi(x, k_dof_recip.m[c]);
 }
 
 // LDS bytes of one DoF workgroup for radius R: colour + blur factor of the
@@ -767,6 +791,8 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B)
     uint32_t *s_col = s_dyn;
     float *s_bf = (float *)(s_dyn + SW * SH);
     const float rmax = (float)R;
+    const FDiv by(J.fdepth - 1.0f);
+    const bool fast = J.fdepth - 1.0f >= 1.0f && J.fdepth - 1.0f <= 0x1p20f;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     // Every load of the halo is issued before the first is used, so the
     // workgroup waits for memory once rather than once per row.
@@ -798,7 +824,7 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B)
             const int c = tx + 64 * m, lx = ox - R + c;
             if (row_ok && c < SW && lx >= 0 && lx < J.V.vw) {
                 s_col[r * SW + c] = cv[k][m];
-                s_bf[r * SW + c] = blur_factor(zv[k][m], J.fdist, J.fdepth, rmax);
+                s_bf[r * SW + c] = blur_factor(zv[k][m], J.fdist, J.fdepth, rmax, by, fast);
             }
         }
     }
@@ -842,8 +868,8 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B)
                 }
             }
             if (count)
-                out = (uint32_t)(uint8_t)(b / count) | ((uint32_t)(uint8_t)(g / count) << 8) |
-                      ((uint32_t)(uint8_t)(r / count) << 16) | (255u << 24);
+                out = (uint32_t)(uint8_t)div_count(b, count) | ((uint32_t)(uint8_t)div_count(g, count) << 8) |
+                      ((uint32_t)(uint8_t)div_count(r, count) << 16) | (255u << 24);
         }
         J.surface[(int64_t)(y + J.V.vy) * J.surface_pitch + (x + J.V.vx)] = out;
     }
@@ -865,22 +891,16 @@ __global__ void k_fill32(uint32_t *p, uint64_t n, uint32_t w)
             return e_;                                                         \
     } while (0)
 
-hipError_t launch_clear(const BatchDev &B, uint32_t w0, uint32_t n_words, hipStream_t s)
-{
-    uint32_t n = B.n_tiles > n_words ? B.n_tiles : n_words;
-    LAUNCH_CHECK(k_clear, dim3((n + 255) / 256), dim3(256), 0, s, B, w0, n_words);
-    return hipSuccess;
-}
 hipError_t launch_world(const WorldArgs &W, hipStream_t s)
 {
     if (W.n_vertices)
         LAUNCH_CHECK(k_world, dim3((W.n_vertices + 255) / 256), dim3(256), 0, s, W);
     return hipSuccess;
 }
-hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, hipStream_t s)
+hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, uint32_t w0, uint32_t n_words, hipStream_t s)
 {
-    if (blocks)
-        LAUNCH_CHECK(k_vertex, dim3(blocks), dim3(256), 0, s, B);
+    // at least one block: the kernel also zeroes the counters
+    LAUNCH_CHECK(k_vertex, dim3(blocks ? blocks : 1), dim3(256), 0, s, B, w0, n_words);
     return hipSuccess;
 }
 hipError_t launch_mark(const BatchDev &B, uint32_t blocks, hipStream_t s)

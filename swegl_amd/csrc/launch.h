@@ -4,9 +4,8 @@
 #include "jobs.h"
 
 namespace sw {
-hipError_t launch_clear(const BatchDev &B, uint32_t w0, uint32_t n_words, hipStream_t s);
 hipError_t launch_world(const WorldArgs &W, hipStream_t s);
-hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, hipStream_t s);
+hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, uint32_t w0, uint32_t n_words, hipStream_t s);
 hipError_t launch_mark(const BatchDev &B, uint32_t blocks, hipStream_t s);
 hipError_t launch_bin_count(const BatchDev &B, uint32_t blocks, hipStream_t s);
 hipError_t launch_scan(const BatchDev &B, hipStream_t s);

@@ -6,28 +6,41 @@
 
 namespace sw {
 
-// (float)pow((double)s, 32) (pixel_shaders.cpp:193): s^32 in double-double
-// by five exact squarings (two-product through fma), rounded once to double
-// and then to float, as glibc pow followed by the float store does.
+// (float)pow((double)s, 32) (pixel_shaders.cpp:193): glibc's pow rounds
+// s^32 to double, the store rounds that to float.  Five plain double
+// squarings give s^32 within 31 double-rounding errors (relative 2^-48);
+// when every value within 2^-46 of it rounds to the same float, that float
+// is the answer.  Otherwise (values near a float rounding boundary) s^32 is
+// recomputed in double-double by exact squarings (two-product through fma)
+// and rounded once to double and then to float.
 SW_HD float pow32(float s)
 {
-    double hi = (double)s, lo = 0.0;
+    double p = (double)s;
+    p *= p;
+    p *= p;
+    p *= p;
+    p *= p;
+    p *= p;
+    const float lo = (float)(p * (1.0 - 0x1p-46)), hi = (float)(p * (1.0 + 0x1p-46));
+    if (lo == hi)
+        return lo;
+    double h = (double)s, l = 0.0;
     for (int i = 0; i < 5; i++) {
-        double p = hi * hi;
-        double e = fma(hi, hi, -p);
-        e += 2.0 * hi * lo;
-        double h2 = p + e;
-        lo = e - (h2 - p);
-        hi = h2;
+        double q = h * h;
+        double e = fma(h, h, -q);
+        e += 2.0 * h * l;
+        double h2 = q + e;
+        l = e - (h2 - q);
+        h = h2;
     }
-    return (float)(hi + lo);
+    return (float)(h + l);
 }
 
 struct Light4 { float x, y, z, I; };
 
 // pixel_shader_lights_phong::shade (pixel_shaders.cpp:159-205)
-SW_HD int32_t phong(f3 center, f3 normal, f3 cam, float ambient, f3 sun, float sun_I,
-                    const Light4 *lights, int n_lights)
+SW_D int32_t phong(f3 center, f3 normal, f3 cam, float ambient, f3 sun, float sun_I,
+                   const Light4 *lights, int n_lights)
 {
     f3 camv = normalize(cam - center);
     float fs = -dot(normal, sun);
@@ -64,23 +77,34 @@ SW_HD int32_t phong(f3 center, f3 normal, f3 cam, float ambient, f3 sun, float s
 // pixel_shader_texture_bilinear::shade (pixel_shaders.cpp:348-384).  The
 // reference reads outside the bitmap for coordinates that wrap to a negative
 // index (UB); here such a tap is clamped and counted.
-SW_HD uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
+// x % m (C remainder); a mask when m is a power of two and x >= 0.
+SW_D int rem(int x, int m)
+{
+    if (x >= 0 && (m & (m - 1)) == 0)
+        return x & (m - 1);
+    return x % m;
+}
+
+SW_D uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
 {
     float v = t.x;
     float u = t.y;
-    float u1 = (float)((double)u - 0.5);   // float - double literal, stored to float
-    float u2 = (float)((double)u + 0.5);
-    float v1 = (float)((double)v - 0.5);
-    float v2 = (float)((double)v + 0.5);
+    // (float)((double)u - 0.5), float minus double literal stored to float,
+    // equals the float operation u - 0.5f: the double difference is exact
+    // whenever |u| >= 2^-30, and below that both round to -0.5 (or +0.5).
+    float u1 = u - 0.5f;
+    float u2 = u + 0.5f;
+    float v1 = v - 0.5f;
+    float v2 = v + 0.5f;
     u = floorf(u2);
     v = floorf(v2);
-    int v1m = wrap_add(cvtt(v1), th) % th;
+    int v1m = rem(wrap_add(cvtt(v1), th), th);
     int v2m = v1m + 1;
     if (v2m == th)
         v2m = 0;
     v1m *= tw;
     v2m *= tw;
-    int u1m = wrap_add(cvtt(u1), tw) % tw;
+    int u1m = rem(wrap_add(cvtt(u1), tw), tw);
     int u2m = u1m + 1;
     if (u2m == tw)
         u2m = 0;
@@ -115,9 +139,12 @@ SW_HD uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
 }
 
 // pixel_shader_light_and_texture::shade combine (pixel_shaders.hpp:159-178)
-SW_HD uint32_t combine(uint32_t color, int32_t light_i)
+SW_D uint32_t combine(uint32_t color, int32_t light_i)
 {
-    float light = (float)((double)light_i / 65536.0);
+    // (float)((double)light_i / 65536.0): the double quotient is exact (a
+    // power-of-two scale of an int), so this is the int rounded to float and
+    // scaled by 2^-16, which the float operations below reproduce exactly.
+    float light = (float)light_i * 0x1p-16f;
     uint32_t out = color & 0xFF000000u;
     if (light < 1) {
 #pragma unroll
@@ -128,10 +155,11 @@ SW_HD uint32_t combine(uint32_t color, int32_t light_i)
     } else {
         light = sqrtf(light);
         light = sqrtf(light);
+        const FDiv by_light(light);   // 1 <= light <= 32768^(1/4); 0 <= 255 - c <= 255
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             uint32_t c = (color >> (8 * k)) & 0xFF;
-            uint8_t r = (uint8_t)(255 - to_u8((float)(255 - (int)c) / light));
+            uint8_t r = (uint8_t)(255 - to_u8(by_light((float)(255 - (int)c))));
             out |= (uint32_t)r << (8 * k);
         }
     }
@@ -152,7 +180,7 @@ struct ScanShade {
     f2 t_left, t_dir;
 };
 
-SW_HD ScanShade prepare_scanline(const TriAttr &A, bool lower, bool lr, float pl, float pr)
+SW_D ScanShade prepare_scanline(const TriAttr &A, bool lower, bool lr, float pl, float pr)
 {
     f3 vleft, vright, vld, vrd, nleft, nright, nld, nrd;
     f2 sst, sstd;
