@@ -110,7 +110,7 @@ struct StageSlot {
 // context; every captured graph owns a set, since their addresses are baked
 // into the graph's kernel arguments.
 struct Resources {
-    DevBuf jobs, tile_count, tile_offset, tile_cursor, bins, ctr;
+    DevBuf jobs, tile_count, tile_offset, bins, ctr;
 };
 
 // A captured launch sequence.  Only the resource addresses and grid sizes
@@ -955,8 +955,8 @@ static int size_resources(Resources &R, std::vector<Wave> &waves)
         cap = std::max(cap, W.cap);
     }
     int rc;
-    if ((rc = R.jobs.ensure(job_bytes)) || (rc = R.tile_count.ensure((size_t)tiles * 4 + 4)) ||
-        (rc = R.tile_offset.ensure((size_t)tiles * 4 + 8)) || (rc = R.tile_cursor.ensure((size_t)tiles * 4 + 4)) ||
+    if ((rc = R.jobs.ensure(job_bytes)) || (rc = R.tile_count.ensure((size_t)tiles * SW_TILE_PAD * 4 + 8)) ||
+        (rc = R.tile_offset.ensure((size_t)tiles * 4 + 8)) ||
         (rc = R.bins.ensure(cap * 4)) || (rc = R.ctr.ensure((size_t)n_ctr * 4)))
         return rc;
     return SWEGL_OK;
@@ -972,7 +972,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     B.n_tiles = W.tiles;
     B.tile_count = R.tile_count.as<uint32_t>();
     B.tile_offset = R.tile_offset.as<uint32_t>();
-    B.tile_cursor = R.tile_cursor.as<uint32_t>();
+    B.tile_cursor = R.tile_count.as<uint32_t>() + 1;
     B.bins = R.bins.as<uint32_t>();
     B.bin_capacity = (uint32_t)W.cap;
     B.ctr = R.ctr.as<uint32_t>();

@@ -96,13 +96,19 @@ struct WorldArgs {
     float4 *world, *nworld;
 };
 
+// Tile counters: tile g's bin count at tile_count[g * SW_TILE_PAD] and its
+// fill cursor one word later, a 128-byte line per tile, so the binning
+// atomics of different tiles spread over all memory channels instead of
+// queueing on the few that a dense array maps to.
+#define SW_TILE_PAD 32
+
 struct BatchDev {
     const JobDev *jobs;
     uint32_t n_jobs;
     uint32_t n_tiles;         // total tiles over all jobs
-    uint32_t *tile_count;     // [n_tiles]
+    uint32_t *tile_count;     // [n_tiles * SW_TILE_PAD], see SW_TILE_PAD
     uint32_t *tile_offset;    // [n_tiles + 1]
-    uint32_t *tile_cursor;    // [n_tiles]
+    uint32_t *tile_cursor;    // tile_count + 1
     uint32_t *bins;           // [capacity]
     uint32_t bin_capacity;
     uint32_t big_tiles;       // per-triangle bin limit; larger ones go to the big list
@@ -112,12 +118,15 @@ struct BatchDev {
 
 // Counter block: [0, 1024) statistics as 64 slots x 8 u64 (slot = block
 // index mod 64, so no address takes one atomic per block; the host sums the
-// slots), [1024, 1088) control words, [1088 + j) big-list length of job j.
+// slots), then the raster's tile queues (one per 256-byte line), the total
+// bin entry count, and the big-list length of each job j at SW_CTR_BIG0 + j.
 #define SW_STAT_SLOTS 64
 #define SW_CTR_STATS 0
 #define SW_CTR_CONTROL 1024
-#define SW_CTR_TOTAL (SW_CTR_CONTROL + 1)
-#define SW_CTR_BIG0 (SW_CTR_CONTROL + 64)
+#define SW_RASTER_QUEUES 8
+#define SW_CTR_QUEUE0 SW_CTR_CONTROL                              // queue k at + 64k (own 256 B)
+#define SW_CTR_TOTAL (SW_CTR_CONTROL + 64 * SW_RASTER_QUEUES)
+#define SW_CTR_BIG0 (SW_CTR_TOTAL + 64)
 // statistic indices within a slot
 #define SW_ST_FRAGMENTS 0
 #define SW_ST_NAN 1

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_vertex(BatchDev B, uint32_t w0, uint32_
 {
     const uint32_t gi = blockIdx.x * 256u + threadIdx.x, stride = gridDim.x * 256u;
     for (uint32_t k = gi; k < B.n_tiles; k += stride)
-        B.tile_count[k] = 0;
+        B.tile_count[k * SW_TILE_PAD] = 0;
     for (uint32_t k = gi; k < n_words; k += stride)
         B.ctr[w0 + k] = 0;
     const JobDev &J = B.jobs[find_job<JV>(B, blockIdx.x)];
@@ -378,12 +378,17 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
             r = make_uint2((uint32_t)tx0 | ((uint32_t)ty0 << 16), (uint32_t)tx1 | ((uint32_t)ty1 << 16));
             ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
             if (ntiles > (int)B.big_tiles) {
-                J.big[atomicAdd(&B.ctr[SW_CTR_BIG0 + jidx], 1u)] = t;
                 ntiles = 0;
                 big = 1;
             }
         }
         J.rect[t] = r;
+    }
+    // big-list append, one atomic per wave (all lanes of a block share the job)
+    {
+        const uint32_t pos = wave_add<true>(B.ctr + SW_CTR_BIG0, (uint32_t)jidx, big != 0);
+        if (big)
+            J.big[pos] = t;
     }
     // one aggregated atomic per distinct tile per wave and round
     uint32_t entries = 0;
@@ -395,7 +400,7 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
             key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
             entries++;
         }
-        wave_add<false>(B.tile_count, key, act);
+        wave_add<false>(B.tile_count, key * SW_TILE_PAD, act);
     }
     if (!B.stats)
         return;
@@ -421,35 +426,70 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
 }
 
 // ----------------------------------------------------------- k_scan
-// Exclusive scan of tile counts by one block of 1024 threads, each owning a
-// contiguous chunk.
+// Exclusive scan of the tile counts by one block of 1024 threads, in passes
+// of 1024 x SCAN_PER tiles: each thread loads its SCAN_PER consecutive
+// counts at once (one memory round trip per pass), the block scans the
+// per-thread sums (shuffles within each wave, then the 16 wave totals), and
+// each thread writes its tiles' offsets and fill cursors.
+#define SCAN_PER 8
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d);
+        if (lane >= d)
+            v += o;
+    }
+    return v;
+}
+
 __global__ __launch_bounds__(1024) void k_scan(BatchDev B)
 {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t s_carry;
     const uint32_t n = B.n_tiles;
-    const uint32_t chunk = (n + 1023) / 1024;
-    const uint32_t a = threadIdx.x * chunk;
-    const uint32_t b = min(n, a + chunk);
-    uint32_t s = 0;
-    for (uint32_t i = a; i < b; i++)
-        s += B.tile_count[i];
-    part[threadIdx.x] = s;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0)
+        s_carry = 0;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0;
+    for (uint32_t base = 0; base < n; base += 1024u * SCAN_PER) {
+        const uint32_t i0 = base + threadIdx.x * SCAN_PER;
+        uint32_t v[SCAN_PER], sum = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++) {
+            v[k] = i0 + k < n ? B.tile_count[(i0 + k) * SW_TILE_PAD] : 0u;
+            sum += v[k];
+        }
+        const uint32_t incl = wave_incl_scan(sum);
+        if (lane == 63)
+            wsum[wave] = incl;
         __syncthreads();
-        part[threadIdx.x] += v;
+        if (wave == 0) {
+            const uint32_t w = lane < 16 ? wsum[lane] : 0u;
+            const uint32_t wi = wave_incl_scan(w);
+            if (lane < 16)
+                wsum[lane] = wi - w;   // exclusive prefix of the wave totals
+        }
+        __syncthreads();
+        const uint32_t carry = s_carry;
+        uint32_t run = carry + wsum[wave] + incl - sum;
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++) {
+            if (i0 + k < n) {
+                B.tile_offset[i0 + k] = run;
+                B.tile_cursor[(i0 + k) * SW_TILE_PAD] = run;
+                run += v[k];
+            }
+        }
+        __syncthreads();   // everyone has read s_carry and wsum
+        if (threadIdx.x == 1023)
+            s_carry = run;   // the last thread's running sum ends the pass
         __syncthreads();
     }
-    uint32_t run = part[threadIdx.x] - s;
-    for (uint32_t i = a; i < b; i++) {
-        B.tile_offset[i] = run;
-        B.tile_cursor[i] = run;
-        run += B.tile_count[i];
-    }
-    if (threadIdx.x == 1023) {
-        B.tile_offset[n] = part[1023];
-        B.ctr[SW_CTR_TOTAL] = part[1023];   // total bin entries
+    if (threadIdx.x == 0) {
+        B.tile_offset[n] = s_carry;
+        B.ctr[SW_CTR_TOTAL] = s_carry;   // total bin entries
     }
 }
 
@@ -478,7 +518,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B)
             const int w = tx1 - tx0 + 1;
             key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
         }
-        const uint32_t pos = wave_add<true>(B.tile_cursor, key, act);
+        const uint32_t pos = wave_add<true>(B.tile_cursor, key * SW_TILE_PAD, act);
         if (act && pos < B.bin_capacity)
             B.bins[pos] = t;
     }
@@ -639,7 +679,7 @@ __device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *k
     uint32_t frags = 0, nans = 0, oob = 0;
     // Replay the tile's binned triangles, then the job's big triangles whose
     // rectangle touches this tile, one per lane.
-    const uint32_t nb = B.tile_count[g], off = B.tile_offset[g];
+    const uint32_t nb = B.tile_count[g * SW_TILE_PAD], off = B.tile_offset[g];
     const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
     for (uint32_t e = threadIdx.x; e < nb + nbig; e += 256) {
         uint32_t t;
@@ -696,15 +736,42 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
     __shared__ uint32_t s_tile;
     if (threadIdx.x < 3)
         s_cnt[threadIdx.x] = 0;
-    // first tile by block index, the rest from the queue as workgroups free up
-    uint32_t g = blockIdx.x;
-    while (g < B.n_tiles) {
-        raster_tile<TW, TH>(B, g, keys, s_cnt);
-        __syncthreads();   // tile done with keys before the next one clears them
-        if (threadIdx.x == 0)
-            s_tile = gridDim.x + atomicAdd(&B.ctr[SW_CTR_CONTROL], 1u);
+    // Tiles come from SW_RASTER_QUEUES queues, each owning a contiguous run
+    // of tiles and counting on its own cache line: same-address atomics are
+    // serialised where they execute, so one shared queue would make every
+    // claim wait for all others.  A workgroup drains its home queue (block
+    // index mod 8, which is also the XCD the hardware deals consecutive
+    // workgroups to, so neighbouring tiles share an L2), then the others in
+    // turn; a queue seen exhausted is never asked again, so every workgroup
+    // stops after at most n_tiles + 8 claims in total.
+    const uint32_t T = B.n_tiles, home = blockIdx.x % SW_RASTER_QUEUES;
+    uint32_t tried = 0;   // thread 0 only: queues found exhausted
+    for (;;) {
+        if (threadIdx.x == 0) {
+            uint32_t tile = 0xFFFFFFFFu;
+            while (tried < SW_RASTER_QUEUES) {
+                const uint32_t k = (home + tried) % SW_RASTER_QUEUES;
+                const uint32_t lo = (uint32_t)((uint64_t)T * k / SW_RASTER_QUEUES);
+                const uint32_t hi = (uint32_t)((uint64_t)T * (k + 1) / SW_RASTER_QUEUES);
+                uint32_t *q = &B.ctr[SW_CTR_QUEUE0 + 64 * k];
+                // counters only grow, so a stale read never skips a live queue
+                if (lo + __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < hi) {
+                    const uint32_t i = atomicAdd(q, 1u);
+                    if (lo + i < hi) {
+                        tile = lo + i;
+                        break;
+                    }
+                }
+                tried++;
+            }
+            s_tile = tile;
+        }
         __syncthreads();
-        g = s_tile;
+        const uint32_t g = s_tile;
+        if (g == 0xFFFFFFFFu)
+            break;
+        raster_tile<TW, TH>(B, g, keys, s_cnt);
+        __syncthreads();   // tile done with keys and s_tile before the next claim
     }
     __syncthreads();
     if (B.stats && threadIdx.x == 0) {
