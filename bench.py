@@ -288,6 +288,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tile", default=None, help="WxH raster tile, e.g. 32x32")
+    ap.add_argument("--raster-groups", type=int, default=0, help="raster workgroups per CU (0 = tuned)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
@@ -322,6 +323,8 @@ def main():
     if args.tile:
         tw, th = (int(x) for x in args.tile.lower().split("x"))
         ctx.set_tile(tw, th)
+    if args.raster_groups:
+        ctx.set_raster_groups(args.raster_groups)
     info = ctx.info()
     ds = DeviceScene(ctx, SceneDescHolder(scene))
     surf = Target(ctx, A.TARGET_COLOR, vp.m_screen.w, vp.m_screen.h)

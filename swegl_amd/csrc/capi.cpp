@@ -858,7 +858,7 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
             (rc = vs.big.ensure(T * 4)))
             return rc;
         const bool dof = v.post_shader != SWEGL_POST_NULL;
-        if (dof && (rc = vs.dof.ensure((size_t)v.w * v.h * 4)))
+        if (dof && (rc = vs.dof.ensure((size_t)v.w * v.h * 5)))   // colour, then one code byte per pixel
             return rc;
         JobDev &J = W.jd[j];
         memset(&J, 0, sizeof J);
@@ -899,8 +899,10 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         J.V.cam = f3{v.camera_position[0], v.camera_position[1], v.camera_position[2]};
         J.surface = v.color->buf.as<uint32_t>();
         J.surface_pitch = v.color->w;
+        J.dofc = nullptr;
         if (dof) {
             J.out_color = vs.dof.as<uint32_t>();
+            J.dofc = (uint8_t *)vs.dof.p + (size_t)v.w * v.h * 4;
             J.out_pitch = v.w;
             J.out_ox = v.x;
             J.out_oy = v.y;

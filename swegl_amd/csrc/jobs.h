@@ -79,6 +79,7 @@ struct JobDev {
     float *zbuf;              // m_zbuffer, w*h, viewport relative
     uint32_t *tid;            // optional winning draw sequence, w*h
     int32_t post;             // SWEGL_POST_*
+    uint8_t *dofc;            // DoF: per pixel (blur factor != 0) | radius << 1, w*h
     float fdist, fdepth;
     int32_t rmax;
     // ---- tiles

@@ -524,6 +524,31 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B)
     }
 }
 
+// `by` divides by fdepth - 1 (the same for the whole job); `fast` when
+// 1 <= fdepth - 1 <= 2^20, where FDiv's quotient equals '/' (the numerator
+// t - 1 lies in [2^-23, fdepth - 1] on that branch).
+__device__ __forceinline__ float blur_factor(float z, float fdist, float fdepth, float rmax, const FDiv &by,
+                                             bool fast)
+{
+    float t = fabsf(fdist - z);
+    // inv_lerp_clipped(1, fdepth, t)
+    float x;
+    if (1.0f == fdepth)
+        x = 0.5f;
+    else if (t <= 1.0f)
+        x = 0.0f;
+    else if (t >= fdepth)
+        x = 1.0f;
+    else
+        x = fast ? by(t - 1.0f) : (t - 1.0f) / (fdepth - 1.0f);
+    // lerp_clipped(0, rmax, x)
+    if (x <= 0)
+        return 0.0f;
+    if (x >= rmax)
+        return rmax;
+    return 0.0f + rmax * x;
+}
+
 // ------------------------------------------------------------- k_raster
 // One workgroup per screen tile.  Phase 1 replays every binned sub-triangle's
 // DDA over the tile's rows and columns (exact float sequence) and keeps the
@@ -716,6 +741,21 @@ __device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *k
         if (J.tid)
             J.tid[zo] = seq;
     }
+    // The depth-of-field pass reads this code instead of z.  A loop of its
+    // own keeps its registers out of the shading loop's peak.
+    if (J.dofc) {
+        const FDiv dof_by(J.fdepth - 1.0f);
+        const bool dof_fast = J.fdepth - 1.0f >= 1.0f && J.fdepth - 1.0f <= 0x1p20f;
+        for (int i = threadIdx.x; i < TW * TH; i += 256) {
+            const int x = tr.x0 + (i % TW), y = tr.y0 + (i / TW);
+            if (x >= tr.x1 || y >= tr.y1)
+                continue;
+            const float z = __uint_as_float((uint32_t)(keys[i] >> 32));   // max_z where empty
+            const float bf = blur_factor(z, J.fdist, J.fdepth, (float)J.rmax, dof_by, dof_fast);
+            const int radius = min(max(cvtt(bf), 0), J.rmax);   // bf is in [0, rmax] for finite inputs
+            J.dofc[(int64_t)(y - J.V.vy) * J.V.vw + (x - J.V.vx)] = (uint8_t)((bf != 0.0f ? 1 : 0) | (radius << 1));
+        }
+    }
     if (B.stats) {
         if (frags)
             atomicAdd(&s_cnt[0], frags);
@@ -728,8 +768,10 @@ __device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *k
 
 // Persistent: each workgroup takes tiles from the batch's work queue until
 // the queue passes the last tile, so tiles of many jobs share the machine.
+// At most 128 VGPRs, so four waves fit on each SIMD (512 registers a lane);
+// the shading path needs about that many and one more halves the overlap.
 template <int TW, int TH>
-__global__ __launch_bounds__(256) void k_raster(BatchDev B)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_raster(BatchDev B)
 {
     __shared__ unsigned long long keys[TW * TH];
     __shared__ uint32_t s_cnt[3];
@@ -788,31 +830,6 @@ __global__ __launch_bounds__(256) void k_raster(BatchDev B)
 // Depth of field (build spec of post_shader_depth_box, post_shaders.hpp:51-131
 // with remap_clipped, lerp.hpp:24-43): 64x16 output pixels per block, colour
 // and blur factor of the tile plus an R_max halo staged in LDS.
-// `by` divides by fdepth - 1 (the same for the whole job); `fast` when
-// 1 <= fdepth - 1 <= 2^20, where FDiv's quotient equals '/' (the numerator
-// t - 1 lies in [2^-23, fdepth - 1] on that branch).
-__device__ __forceinline__ float blur_factor(float z, float fdist, float fdepth, float rmax, const FDiv &by,
-                                             bool fast)
-{
-    float t = fabsf(fdist - z);
-    // inv_lerp_clipped(1, fdepth, t)
-    float x;
-    if (1.0f == fdepth)
-        x = 0.5f;
-    else if (t <= 1.0f)
-        x = 0.0f;
-    else if (t >= fdepth)
-        x = 1.0f;
-    else
-        x = fast ? by(t - 1.0f) : (t - 1.0f) / (fdepth - 1.0f);
-    // lerp_clipped(0, rmax, x)
-    if (x <= 0)
-        return 0.0f;
-    if (x >= rmax)
-        return rmax;
-    return 0.0f + rmax * x;
-}
-
 // Exact floor(x / c) for 1 <= c <= 1024 and x < 2^18 (a tap count and a
 // sum of 8-bit channels) as a high multiply by m = ceil(2^32 / c): the
 // error x * (m * c - 2^32) / 2^32 < 2^18 * 2^10 / 2^32 stays below 1/c.
@@ -832,17 +849,18 @@ __device__ __forceinline__ uint32_t div_count(uint32_t x, uint32_t c)
     return c == 1 ? x : __umulhi(x, k_dof_recip.m[c]);
 }
 
-// LDS bytes of one DoF workgroup for radius R: colour + blur factor of the
+// LDS bytes of one DoF workgroup for radius R: colour and code of the
 // output tile plus an R halo on every side.
 __host__ __device__ inline size_t dof_lds_bytes(int R)
 {
-    return (size_t)(SW_DOF_TW + 2 * R) * (SW_DOF_TH + 2 * R) * 8;
+    return (size_t)(SW_DOF_TW + 2 * R) * (SW_DOF_TH + 2 * R) * 5 + 4;
 }
 
 // 256 threads as 64 columns x 4 rows.  Consecutive workgroups run on
 // different XCDs (round robin over 8), so the job's tiles are dealt out in
 // 8 contiguous runs, one per XCD: neighbouring tiles, whose halos overlap,
-// then share that XCD's L2.
+// then share that XCD's L2.  The raster stored, per pixel, whether its blur
+// factor is nonzero (an "unfocused" tap) and its clamped radius.
 __global__ __launch_bounds__(256) void k_dof(BatchDev B)
 {
     extern __shared__ uint32_t s_dyn[];
@@ -856,42 +874,42 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B)
     const int R = J.rmax;
     const int SW = SW_DOF_TW + 2 * R, SH = SW_DOF_TH + 2 * R;
     uint32_t *s_col = s_dyn;
-    float *s_bf = (float *)(s_dyn + SW * SH);
-    const float rmax = (float)R;
-    const FDiv by(J.fdepth - 1.0f);
-    const bool fast = J.fdepth - 1.0f >= 1.0f && J.fdepth - 1.0f <= 0x1p20f;
+    uint8_t *s_code = (uint8_t *)(s_dyn + SW * SH);
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     // Every load of the halo is issued before the first is used, so the
     // workgroup waits for memory once rather than once per row.
     constexpr int ROWS = (SW_DOF_TH + 2 * SW_MAX_DOF_RADIUS + 3) / 4;
     constexpr int COLS = (SW_DOF_TW + 2 * SW_MAX_DOF_RADIUS + 63) / 64;
     uint32_t cv[ROWS][COLS];
-    float zv[ROWS][COLS];
+    uint32_t kv[ROWS][COLS];
 #pragma unroll
     for (int k = 0; k < ROWS; k++) {
         const int r = ty + 4 * k, ly = oy - R + r;
         const bool row_ok = r < SH && ly >= 0 && ly < J.V.vh;
         const uint32_t *crow = J.out_color + (int64_t)(ly + J.V.vy - J.out_oy) * J.out_pitch + (J.V.vx - J.out_ox);
-        const float *zrow = J.zbuf + (int64_t)ly * J.V.vw;
+        const uint8_t *krow = J.dofc + (int64_t)ly * J.V.vw;
 #pragma unroll
         for (int m = 0; m < COLS; m++) {
             const int c = tx + 64 * m, lx = ox - R + c;
+            cv[k][m] = 0u;
+            kv[k][m] = 0u;
             if (row_ok && c < SW && lx >= 0 && lx < J.V.vw) {
                 cv[k][m] = crow[lx];
-                zv[k][m] = zrow[lx];
+                kv[k][m] = krow[lx];
             }
         }
     }
+    // Cells outside the viewport keep code 0: a tap there is never an
+    // unfocused pixel, which is what the reference's clipped loops give.
 #pragma unroll
     for (int k = 0; k < ROWS; k++) {
-        const int r = ty + 4 * k, ly = oy - R + r;
-        const bool row_ok = r < SH && ly >= 0 && ly < J.V.vh;
+        const int r = ty + 4 * k;
 #pragma unroll
         for (int m = 0; m < COLS; m++) {
-            const int c = tx + 64 * m, lx = ox - R + c;
-            if (row_ok && c < SW && lx >= 0 && lx < J.V.vw) {
+            const int c = tx + 64 * m;
+            if (r < SH && c < SW) {
                 s_col[r * SW + c] = cv[k][m];
-                s_bf[r * SW + c] = blur_factor(zv[k][m], J.fdist, J.fdepth, rmax, by, fast);
+                s_code[r * SW + c] = (uint8_t)kv[k][m];
             }
         }
     }
@@ -903,36 +921,29 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B)
             continue;
         const int c = (py + R) * SW + (px + R);
         const uint32_t src = s_col[c];
-        const int radius = min(max(cvtt(s_bf[c]), 0), R);   // bf is in [0, R] for finite inputs
+        const int radius = s_code[c] >> 1;
         uint32_t out = src;
         if (radius != 0) {
-            int b = 0, g = 0, r = 0, count = 0;
-            auto tap = [&](int i, int j) {
-                const int k = (j - oy + R) * SW + (i - ox + R);
-                if (s_bf[k] != 0) {
-                    count++;
-                    const uint32_t q = s_col[k];
-                    b += q & 0xFF;
-                    g += (q >> 8) & 0xFF;
-                    r += (q >> 16) & 0xFF;
-                }
+            uint32_t b = 0, g = 0, r = 0, count = 0;
+            auto tap = [&](int k) {   // LDS cell k; adds it when unfocused
+                const uint32_t m = 0u - (uint32_t)(s_code[k] & 1u);
+                const uint32_t qq = s_col[k] & m;
+                count -= m;   // +1 when taken
+                b += qq & 0xFF;
+                g += (qq >> 8) & 0xFF;
+                r += (qq >> 16) & 0xFF;
             };
             if (J.post == 1) {   // box [x-r, x+r) x [y-r, y+r)
-                for (int j = max(0, y - radius); j < min(J.V.vh, y + radius); j++)
-                    for (int i = max(0, x - radius); i < min(J.V.vw, x + radius); i++)
-                        tap(i, j);
+                for (int j = -radius; j < radius; j++)
+                    for (int i = -radius; i < radius; i++)
+                        tap(c + j * SW + i);
             } else {             // 3x3 taps at {-r, 0, r}
-                for (int a = -1; a <= 1; a++) {
-                    const int j = y + a * radius;
-                    if (j < 0 || j >= J.V.vh)
-                        continue;
-                    for (int e = -1; e <= 1; e++) {
-                        const int i = x + e * radius;
-                        if (i < 0 || i >= J.V.vw)
-                            continue;
-                        tap(i, j);
-                    }
-                }
+                const int dr = radius * SW;
+#pragma unroll
+                for (int a = -1; a <= 1; a++)
+#pragma unroll
+                    for (int e = -1; e <= 1; e++)
+                        tap(c + a * dr + e * radius);
             }
             if (count)
                 out = (uint32_t)(uint8_t)div_count(b, count) | ((uint32_t)(uint8_t)div_count(g, count) << 8) |

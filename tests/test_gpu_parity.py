@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
-from swegl_amd import scenes
+from swegl_amd import host, scenes
 from swegl_amd.device import Context
 
 from _helpers import compare, gpu_render
@@ -40,6 +40,21 @@ def test_heightfield_small(ctx, seed, n, w, h, dof):
     got = gpu_render(ctx, scene, vps)[0]
     compare(got, want, colour_tol=2 if dof else 1)
     assert got.stats["fragments"] == want.stats["fragments"]
+
+
+@pytest.mark.parametrize("taps,rmax,fdepth", [("box", 5, 5.0), ("box", 16, 3.0), ("tap9", 16, 1.5),
+                                              ("tap9", 1, 5.0)])
+def test_dof_variants(ctx, taps, rmax, fdepth):
+    """Box and 9-tap DoF, radii 1..16 (16 = the largest halo), focal depths
+    with fdepth - 1 below and above 1 (both division paths)."""
+    scene, vps = scenes.heightfield_scene(9, n=120, width=300, height=200, tex=128, dof=False)
+    vp = vps[0]
+    vp.set_post_shader(host.post_shader_depth_box(2.0, fdepth, vp, max_radius=rmax, taps=taps))
+    want = oracle.render(scene, vps)[0]
+    got = gpu_render(ctx, scene, vps)[0]
+    compare(got, want, colour_tol=2)
+    # blurred pixels exist, so the test exercises the taps
+    assert (want.zbuffer.view(np.uint32) != 0x7F7F7F7F).sum() > 1000
 
 
 def test_heightfield_1m_1080p(ctx):
