@@ -22,6 +22,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# 256 CUs x 4 SIMDs, one wave64 FP32 VALU instruction per SIMD every 4 cycles, 2.4 GHz
+VALU_PEAK_G = 256 * 4 * 2.4 / 4
 METRIC = "frames/s + Gfragments/s, batched vs plain on GPU; p50/p99 frame latency"
 
 
@@ -382,15 +384,28 @@ def main():
     frame_bytes, raster_bytes = algorithmic_bytes(scene, vp, f_vis)
     dom_bytes = raster_bytes if dom[0] == "raster" else None
     achieved = dom_bytes / (dom[1][0] * 1e-3) / 1e9 if dom_bytes else None
-    traffic = None
+    # HBM traffic and VALU instruction count of the dominant kernel per launch,
+    # from the committed rocprofv3 summary of this workload
+    traffic, valu_instr = None, None
     tfile = os.path.join(ROOT, "profiles", "heightfield_1080p_kernels.json")
     if os.path.exists(tfile):
         try:
             with open(tfile) as f:
                 tj = json.load(f)
             traffic = tj.get(dom[0])
+            if dom[0] == "raster":
+                valu_instr = tj.get("raster_valu_instr")
         except (OSError, ValueError):
             traffic = None
+    valu = None
+    if valu_instr:
+        t = dom[1][0] * 1e-3   # live average launch duration, seconds
+        valu = {"kernel": dom[0], "valu_instr_per_launch": valu_instr,
+                "achieved": valu_instr / t / 1e9, "peak": VALU_PEAK_G, "unit": "G wave64 VALU instr/s",
+                "frac": valu_instr / t / 1e9 / VALU_PEAK_G,
+                "lane_instr_per_shaded_pixel": valu_instr * 64 / f_vis,
+                "shaded_pixels_per_s": f_vis / t / 1e9,
+                "shaded_pixels_per_s_at_valu_peak": VALU_PEAK_G * f_vis / valu_instr}
 
     out = {
         "metric": METRIC,
@@ -421,6 +436,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,
                      "traffic": traffic, "algorithmic_bytes": dom_bytes},
+        "valu_roofline": valu,
         "device": info,
     }
     if args.batch:

@@ -1,9 +1,10 @@
 // swegl/render/viewport.hpp -- viewport_t and shader plugin types of the
 // swegl API (reference: swegl/render/viewport.hpp, pixel_shaders.hpp,
-// post_shaders.hpp).  Shader objects are type tags: the GPU path implements
-// pixel_shader_light_and_texture<pixel_shader_lights_phong,
-// pixel_shader_texture_bilinear>, post_shader_t (no-op without
-// transparency) and the depth-of-field post shaders.
+// post_shaders.hpp).  Shader objects are type tags naming the GPU kernel
+// variant that computes them: every pixel shader type of the reference and
+// every pixel_shader_light_and_texture<L, T> of its light and texture types,
+// post_shader_t (no-op without transparency) and the depth-of-field post
+// shaders.
 #pragma once
 
 #include <memory>
@@ -16,27 +17,49 @@
 namespace swegl
 {
 
+// pixel_shader_t shades with the material colour (pixel_shaders.hpp:15-29)
 struct pixel_shader_t {
     virtual ~pixel_shader_t() = default;
-    virtual int gpu_kind() const { return 0; }
+    virtual int gpu_kind() const { return SWEGL_SHADER_DEFAULT; }
 };
 struct pixel_shader_lights_phong : pixel_shader_t {
+    int gpu_kind() const override { return SWEGL_SHADER_LIGHTS_PHONG; }
 };
 struct pixel_shader_lights_flat : pixel_shader_t {
+    int gpu_kind() const override { return SWEGL_SHADER_LIGHTS_FLAT; }
 };
 struct pixel_shader_texture_bilinear : pixel_shader_t {
+    int gpu_kind() const override { return SWEGL_SHADER_TEXTURE_BILINEAR; }
 };
 struct pixel_shader_texture : pixel_shader_t {
+    int gpu_kind() const override { return SWEGL_SHADER_TEXTURE_NEAREST; }
 };
+
+namespace detail
+{
+template <typename L, typename T>
+struct combined_kind;   // only light x texture combinations exist
+template <>
+struct combined_kind<pixel_shader_lights_phong, pixel_shader_texture_bilinear> {
+    static constexpr int value = SWEGL_SHADER_PHONG_BILINEAR;
+};
+template <>
+struct combined_kind<pixel_shader_lights_phong, pixel_shader_texture> {
+    static constexpr int value = SWEGL_SHADER_PHONG_NEAREST;
+};
+template <>
+struct combined_kind<pixel_shader_lights_flat, pixel_shader_texture_bilinear> {
+    static constexpr int value = SWEGL_SHADER_FLAT_BILINEAR;
+};
+template <>
+struct combined_kind<pixel_shader_lights_flat, pixel_shader_texture> {
+    static constexpr int value = SWEGL_SHADER_FLAT_NEAREST;
+};
+}  // namespace detail
 
 template <typename L, typename T>
 struct pixel_shader_light_and_texture : pixel_shader_t {
-    int gpu_kind() const override { return 0; }
-};
-template <>
-struct pixel_shader_light_and_texture<pixel_shader_lights_phong, pixel_shader_texture_bilinear>
-    : pixel_shader_t {
-    int gpu_kind() const override { return SWEGL_SHADER_PHONG_BILINEAR; }
+    int gpu_kind() const override { return detail::combined_kind<L, T>::value; }
 };
 
 struct viewport_t;

@@ -121,10 +121,21 @@ typedef struct swegl_scene_desc {
 
 /* ---- viewport (swegl/render/viewport.hpp:27-62) ------------------------- */
 
-/* pixel shader plugins (swegl/render/pixel_shaders.hpp).  The GPU path
- * implements pixel_shader_light_and_texture<pixel_shader_lights_phong,
- * pixel_shader_texture_bilinear> (pixel_shaders.hpp:121-179). */
-#define SWEGL_SHADER_PHONG_BILINEAR  1
+/* pixel shader plugins (swegl/render/pixel_shaders.hpp): every concrete
+ * shader type of the reference and every pixel_shader_light_and_texture<L, T>
+ * (pixel_shaders.hpp:121-179) combination of its light and texture types.
+ * The shaders that sample a texture index materials[material_id] and so
+ * reject primitives with material_id == -1 (UB in the reference,
+ * pixel_shaders.cpp:213-215, 288-290); the others use default_material. */
+#define SWEGL_SHADER_PHONG_BILINEAR   1   /* light_and_texture<lights_phong, texture_bilinear> */
+#define SWEGL_SHADER_PHONG_NEAREST    2   /* light_and_texture<lights_phong, texture>          */
+#define SWEGL_SHADER_FLAT_BILINEAR    3   /* light_and_texture<lights_flat, texture_bilinear>  */
+#define SWEGL_SHADER_FLAT_NEAREST     4   /* light_and_texture<lights_flat, texture>           */
+#define SWEGL_SHADER_DEFAULT          5   /* pixel_shader_t: the material colour (:28)         */
+#define SWEGL_SHADER_TEXTURE_NEAREST  6   /* pixel_shader_texture alone: the texel             */
+#define SWEGL_SHADER_TEXTURE_BILINEAR 7   /* pixel_shader_texture_bilinear alone               */
+#define SWEGL_SHADER_LIGHTS_FLAT      8   /* pixel_shader_lights_flat alone: the light word    */
+#define SWEGL_SHADER_LIGHTS_PHONG     9   /* pixel_shader_lights_phong alone                   */
 
 /* post shaders (swegl/render/post_shaders.hpp) */
 #define SWEGL_POST_NULL        0   /* post_shader_t, no transparency: no-op (:33-47)   */

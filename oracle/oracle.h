@@ -32,6 +32,8 @@ typedef struct oracle_viewport {
     uint32_t *color64;                 /* w*h or NULL: FP64 re-shade of each  */
                                        /* pixel's final winner, before the    */
                                        /* post shader; 0 where nothing drew   */
+                                       /* (SWEGL_SHADER_PHONG_BILINEAR only)  */
+    int32_t   pixel_shader;            /* SWEGL_SHADER_*                      */
 } oracle_viewport;
 
 typedef struct oracle_stats {
@@ -77,6 +79,8 @@ void oracle_dof(const uint32_t *src, const float *z, int w, int h, float focal_d
                 float focal_depth, int max_radius, int taps9, uint32_t *dst);
 /* x86-64 cvttss2si semantics: out of range / NaN -> INT32_MIN. */
 int32_t oracle_cvtt(float f);
+/* pixel_shader_texture::shade texel fetch (pixel_shaders.cpp:275-281) */
+uint32_t oracle_nearest(const uint32_t *bitmap, int w, int h, float tx, float ty);
 
 #ifdef __cplusplus
 }

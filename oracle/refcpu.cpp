@@ -240,12 +240,35 @@ struct shader {
     const uint32_t *tbitmap;
     int twidth, theight;
     uint64_t texel_oob = 0;
+    /* kind of pixel shader (SWEGL_SHADER_*), split into its light part
+     * (0 none, 1 flat, 2 phong) and texture part (0 the material colour of
+     * pixel_shader_t, 1 nearest, 2 bilinear) */
+    int light_kind = 2, tex_kind = 2;
+    bool combined = true;          /* pixel_shader_light_and_texture<L, T> */
+    uint32_t material_color;       /* pixel_shader_t::color (pixel_shaders.cpp:14-31) */
+    int32_t flat_light;            /* pixel_shader_lights_flat::light, as returned by shade */
 
-    /* pixel_shader_texture_bilinear::prepare_for_primitive (pixel_shaders.cpp:284-302);
-     * material_id == -1 indexes materials[-1] there (UB): rejected upstream. */
+    void set_kind(int kind)
+    {
+        static const int L[10] = {0, 2, 2, 1, 1, 0, 0, 0, 1, 2};
+        static const int T[10] = {0, 2, 1, 2, 1, 0, 1, 2, 0, 0};
+        light_kind = L[kind];
+        tex_kind = T[kind];
+        combined = kind >= 1 && kind <= 4;
+    }
+
+    /* pixel_shader_t::prepare_for_primitive (pixel_shaders.cpp:14-31), then the
+     * texture shaders' (:209-228, :284-302).  A texture shader indexes
+     * materials[material_id], UB for -1: rejected before rendering. */
     void prepare_for_primitive(const primitive &p)
     {
         prim = &p;
+        material_color = p.material_id != -1 ? sc->materials[p.material_id].color : sc->default_material.color;
+        if (tex_kind == 0) {
+            tbitmap = &default_bitmap;
+            twidth = theight = 1;
+            return;
+        }
         const swegl_material_desc &m = sc->materials[p.material_id];
         if (m.texture_idx == -1) {
             default_bitmap = m.color;
@@ -286,6 +309,8 @@ struct shader {
         t2.x *= twidth;
         t2.y *= theight;
         side_long_t_dir = t2 - t0;
+        if (light_kind == 1)
+            flat_light = flat(vs[i0].v_world, vs[i1].v_world, vs[i2].v_world, inverted, vp->cam, *sc);
     }
     void prepare_for_upper_triangle(bool lr)
     {
@@ -352,11 +377,48 @@ struct shader {
     }
     uint32_t shade(float p)
     {
-        uint32_t color = bilinear(tbitmap, twidth, theight, t_left + t_dir * p, &texel_oob);
-        vec3 center = v + vdir * p;
-        vec3 normal = normalize(n + ndir * p);   /* normal_t(vector_t&&) normalises */
-        int32_t light = phong(center, normal, vp->cam, *sc);
-        return combine(color, light);
+        uint32_t color = material_color;
+        if (tex_kind == 1)
+            color = nearest(tbitmap, (uint32_t)twidth, (uint32_t)theight, t_left + t_dir * p);
+        else if (tex_kind == 2)
+            color = bilinear(tbitmap, twidth, theight, t_left + t_dir * p, &texel_oob);
+        if (light_kind == 0)
+            return color;
+        int32_t light;
+        if (light_kind == 1) {
+            light = flat_light;
+        } else {
+            vec3 center = v + vdir * p;
+            vec3 normal = normalize(n + ndir * p);   /* normal_t(vector_t&&) normalises */
+            light = phong(center, normal, vp->cam, *sc);
+        }
+        return combined ? combine(color, light) : (uint32_t)light;
+    }
+
+    /* pixel_shader_texture::shade (pixel_shaders.cpp:275-281): twidth and
+     * theight are unsigned there, so (int)t % twidth is an unsigned
+     * remainder of the int's bits; column from t.x, row from t.y. */
+    static uint32_t nearest(const uint32_t *pc, uint32_t w, uint32_t h, vec2 t)
+    {
+        uint32_t u = (uint32_t)cvtt(t.x) % w;
+        uint32_t v = (uint32_t)cvtt(t.y) % h;
+        return pc[v * w + u];
+    }
+
+    /* pixel_shader_lights_flat::prepare_for_triangle (pixel_shaders.cpp:33-84):
+     * the face normal normal_t(cross(...)) is normalised once, and once more
+     * when negated (unary minus builds a normal_t, points.hpp:171-174); the
+     * centroid ((v0 + v1) + v2) / 3; then the Phong light sum at the centroid,
+     * light = (ambient + sun + lights) * 65536 returned as int. */
+    static int32_t flat(vec3 w0, vec3 w1, vec3 w2, bool inverted, vec3 cam, const scene_m &sc)
+    {
+        vec3 nrm = cross_n(w1 - w0, w2 - w0);
+        if (inverted)
+            nrm = normalize(neg(nrm));
+        vec3 s01 = w0 + w1;
+        vec3 s = s01 + w2;
+        vec3 center = {s.x / 3, s.y / 3, s.z / 3};
+        return phong(center, nrm, cam, sc);
     }
 
     /* pixel_shader_texture_bilinear::shade (pixel_shaders.cpp:348-384) */
@@ -1006,9 +1068,9 @@ int import_scene(const swegl_scene_desc *d, scene_m &sc)
             if (pd.mode != SWEGL_TRIANGLES && pd.mode != SWEGL_TRIANGLE_STRIP &&
                 pd.mode != SWEGL_TRIANGLE_FAN)
                 return SWEGL_ENOTSUP;
-            if (pd.material_id < 0 || (uint32_t)pd.material_id >= d->n_materials)
+            if (pd.material_id < -1 || pd.material_id >= (int)d->n_materials)
                 return SWEGL_EINVAL;
-            int ti = d->materials[pd.material_id].texture_idx;
+            int ti = pd.material_id == -1 ? -1 : d->materials[pd.material_id].texture_idx;
             if (ti < -1 || ti >= (int)d->n_textures)
                 return SWEGL_EINVAL;
             pr.vertices.resize(pd.n_vertices);
@@ -1101,6 +1163,7 @@ void render_viewport(scene_m &sc, viewport_m &vp, const oracle_viewport &ov, ora
     shader sh{};
     sh.sc = &sc;
     sh.vp = &vp;
+    sh.set_kind(ov.pixel_shader);
     shader64 sh64{};
     sh64.sc = &sc;
     sh64.vp = &vp;
@@ -1198,6 +1261,18 @@ int oracle_render(const swegl_scene_desc *scene, oracle_viewport *vps, uint32_t 
             return SWEGL_EINVAL;
         if (o.post_shader < SWEGL_POST_NULL || o.post_shader > SWEGL_POST_DEPTH_TAP9)
             return SWEGL_ENOTSUP;
+        if (o.pixel_shader < SWEGL_SHADER_PHONG_BILINEAR || o.pixel_shader > SWEGL_SHADER_LIGHTS_PHONG)
+            return SWEGL_ENOTSUP;
+        if (o.color64 && o.pixel_shader != SWEGL_SHADER_PHONG_BILINEAR)
+            return SWEGL_ENOTSUP;
+        const bool textured = o.pixel_shader <= SWEGL_SHADER_FLAT_NEAREST ||
+                              o.pixel_shader == SWEGL_SHADER_TEXTURE_NEAREST ||
+                              o.pixel_shader == SWEGL_SHADER_TEXTURE_BILINEAR;
+        if (textured)
+            for (const node &nd : sc.nodes)
+                for (const primitive &pr : nd.prims)
+                    if (pr.material_id == -1)
+                        return SWEGL_EINVAL;   /* materials[-1] in the texture shader */
     }
     for (int r : sc.roots)
         original_to_world(sc, r, kIdentity);
@@ -1308,5 +1383,10 @@ void oracle_dof(const uint32_t *src, const float *z, int w, int h, float focal_d
 }
 
 int32_t oracle_cvtt(float f) { return cvtt(f); }
+
+uint32_t oracle_nearest(const uint32_t *bitmap, int w, int h, float tx, float ty)
+{
+    return shader::nearest(bitmap, (uint32_t)w, (uint32_t)h, {tx, ty});
+}
 
 }  // extern "C"

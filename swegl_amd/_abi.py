@@ -14,6 +14,14 @@ ERRNAMES = {0: "OK", -5: "EIO", -11: "EAGAIN", -12: "ENOMEM", -19: "ENODEV",
 TRIANGLES, TRIANGLE_STRIP, TRIANGLE_FAN = 4, 5, 6
 
 SHADER_PHONG_BILINEAR = 1
+SHADER_PHONG_NEAREST = 2
+SHADER_FLAT_BILINEAR = 3
+SHADER_FLAT_NEAREST = 4
+SHADER_DEFAULT = 5
+SHADER_TEXTURE_NEAREST = 6
+SHADER_TEXTURE_BILINEAR = 7
+SHADER_LIGHTS_FLAT = 8
+SHADER_LIGHTS_PHONG = 9
 POST_NULL, POST_DEPTH_BOX, POST_DEPTH_TAP9 = 0, 1, 2
 
 TARGET_COLOR, TARGET_FLOAT, TARGET_UINT = 0, 1, 2

@@ -84,6 +84,7 @@ struct ViewScratch {
 struct swegl_scene {
     DevBuf px, py, pz, nx, ny, nz, tu, tv, tri, prims, nodes, texels, texs, lights, world, nworld;
     uint32_t n_vertices = 0, n_tris = 0, n_prims = 0, n_lights = 0;
+    bool no_material = false;   // some primitive has material_id -1
     std::vector<PrimDev> h_prims;
     std::vector<swegl_node_desc> h_nodes;
     std::vector<int32_t> roots, children;
@@ -631,13 +632,16 @@ extern "C" int swegl_scene_upload(swegl_ctx *c, const swegl_scene_desc *d, swegl
             const swegl_primitive_desc &p = d->primitives[nd.first_primitive + k];
             if (p.mode != SWEGL_TRIANGLES && p.mode != SWEGL_TRIANGLE_STRIP && p.mode != SWEGL_TRIANGLE_FAN)
                 return fail(SWEGL_ENOTSUP, "primitive mode %d is not a triangle mode", p.mode);
-            if (p.material_id < 0 || (uint32_t)p.material_id >= d->n_materials)
-                return fail(SWEGL_EINVAL,
-                            "material_id %d: the bilinear shader indexes materials[material_id] "
-                            "(pixel_shaders.cpp:288-290)", p.material_id);
-            const swegl_material_desc &m = d->materials[p.material_id];
-            if (m.texture_idx < -1 || m.texture_idx >= (int)d->n_textures)
+            // material_id -1: pixel_shader_t takes default_material (pixel_shaders.cpp:14-31);
+            // the texture shaders would index materials[-1], so renders with them refuse
+            // such a scene (swegl_scene::no_material)
+            if (p.material_id < -1 || p.material_id >= (int)d->n_materials)
+                return fail(SWEGL_EINVAL, "material_id %d out of range", p.material_id);
+            const bool nomat = p.material_id == -1;
+            const swegl_material_desc &m = nomat ? d->default_material : d->materials[p.material_id];
+            if (!nomat && (m.texture_idx < -1 || m.texture_idx >= (int)d->n_textures))
                 return fail(SWEGL_EINVAL, "texture_idx %d out of range", m.texture_idx);
+            s->no_material = s->no_material || nomat;
             if (p.n_vertices && (!p.positions || !p.normals || !p.tex_coords))
                 return fail(SWEGL_EINVAL, "null vertex arrays");
             for (uint32_t i = 0; i < p.n_indices; i++)
@@ -649,9 +653,9 @@ extern "C" int swegl_scene_upload(swegl_ctx *c, const swegl_scene_desc *d, swegl
             pd.n_vertices = p.n_vertices;
             pd.first_tri = (uint32_t)(tri.size() / 3);
             pd.node = ni;
-            pd.tex = m.texture_idx;
+            pd.tex = nomat ? -1 : m.texture_idx;
             pd.color = m.color;
-            pd.double_sided = m.double_sided ? 1 : 0;
+            pd.double_sided = (!nomat && m.double_sided) ? 1 : 0;   // renderer.cpp:89-90
             const uint32_t base = pd.first_vertex;
             for (uint32_t v = 0; v < p.n_vertices; v++) {
                 px.push_back(p.positions[3 * v]);
@@ -802,8 +806,8 @@ static int check_viewport(const swegl_viewport_desc &v)
         return fail(SWEGL_EINVAL, "triangle id target must hold w*h uints");
     if (v.transparency_layer_count > 0)
         return fail(SWEGL_ENOTSUP, "transparency layers are not on the GPU path");
-    if (v.pixel_shader != SWEGL_SHADER_PHONG_BILINEAR)
-        return fail(SWEGL_ENOTSUP, "pixel shader %d is not on the GPU path", v.pixel_shader);
+    if (v.pixel_shader < SWEGL_SHADER_PHONG_BILINEAR || v.pixel_shader > SWEGL_SHADER_LIGHTS_PHONG)
+        return fail(SWEGL_ENOTSUP, "pixel shader %d is not a swegl shader type", v.pixel_shader);
     if (v.post_shader < SWEGL_POST_NULL || v.post_shader > SWEGL_POST_DEPTH_TAP9)
         return fail(SWEGL_EINVAL, "post shader %d (the reference dereferences an unset post shader)",
                     v.post_shader);
@@ -838,6 +842,7 @@ struct Wave {
     std::vector<JobDev> jd;
     uint32_t vblk = 0, tblk = 0, dblk = 0, tiles = 0, n_ctr = 0;
     int rmax = 0;             // largest DoF radius (sizes the DoF LDS)
+    bool generic = false;     // some job uses a shader other than Phong + bilinear
     uint64_t cap = 0;
     size_t job_offset = 0;   // byte offset of this wave's table in Resources::jobs
 };
@@ -914,6 +919,8 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         }
         J.zbuf = v.depth->buf.as<float>();
         J.tid = v.triangle_id ? v.triangle_id->buf.as<uint32_t>() : nullptr;
+        J.shader = v.pixel_shader;
+        W.generic = W.generic || v.pixel_shader != SWEGL_SHADER_PHONG_BILINEAR;
         J.post = v.post_shader;
         J.fdist = v.focal_distance;
         J.fdepth = v.focal_depth;
@@ -996,7 +1003,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     STAGE("bin_count", launch_bin_count(B, W.tblk, st));
     STAGE("scan", launch_scan(B, st));
     STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
-    STAGE("raster", launch_raster(B, c->tw, c->th, groups, st));
+    STAGE("raster", launch_raster(B, c->tw, c->th, groups, W.generic, st));
     STAGE("dof", launch_dof(B, W.dblk, W.rmax, st));
 #undef STAGE
     return SWEGL_OK;
@@ -1011,6 +1018,12 @@ static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::ve
         int rc = check_viewport(*j.vp);
         if (rc)
             return rc;
+        const int k = j.vp->pixel_shader;
+        const bool textured = k <= SWEGL_SHADER_FLAT_NEAREST || k == SWEGL_SHADER_TEXTURE_NEAREST ||
+                              k == SWEGL_SHADER_TEXTURE_BILINEAR;
+        if (textured && j.scene->no_material)
+            return fail(SWEGL_EINVAL, "a primitive has material_id -1, which the texture shaders index "
+                                      "(materials[-1], pixel_shaders.cpp:213-215, 288-290)");
     }
     // waves: a job conflicting with an earlier job of the current wave (same
     // surface region or planes) starts a new wave, which keeps the
@@ -1083,7 +1096,7 @@ static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
     for (const Wave &W : waves) {
         k += "|" + std::to_string(W.jd.size()) + ":" + std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
              "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap) + "," +
-             std::to_string(W.rmax);
+             std::to_string(W.rmax) + (W.generic ? "g" : "");
     }
     return k;
 }

@@ -78,6 +78,7 @@ struct JobDev {
     int32_t surface_pitch;    // words per row
     float *zbuf;              // m_zbuffer, w*h, viewport relative
     uint32_t *tid;            // optional winning draw sequence, w*h
+    int32_t shader;           // SWEGL_SHADER_*
     int32_t post;             // SWEGL_POST_*
     uint8_t *dofc;            // DoF: per pixel (blur factor != 0) | radius << 1, w*h
     float fdist, fdepth;

@@ -639,8 +639,21 @@ __device__ void vertex_attr(const JobDev &J, const NodeDev &N, const SVert &v, f
     n = normal_world(N.RS, nobj);
 }
 
+// GENERIC: the job's shader is any SWEGL_SHADER_* kind, decoded into a light
+// part (0 none, 1 flat, 2 phong) and a texture part (0 the material colour,
+// 1 nearest, 2 bilinear); otherwise it is SWEGL_SHADER_PHONG_BILINEAR, the
+// kernel variant the common case runs without the other branches.
+template <bool GENERIC>
 __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uint32_t &oob)
 {
+    int lk = 2, tk = 2;
+    bool combined = true;
+    if (GENERIC) {
+        const int k = J.shader;
+        lk = (k == 1 || k == 2 || k == 9) ? 2 : (k == 3 || k == 4 || k == 8) ? 1 : 0;
+        tk = (k == 1 || k == 3 || k == 7) ? 2 : (k == 2 || k == 4 || k == 6) ? 1 : 0;
+        combined = k >= 1 && k <= 4;
+    }
     const uint32_t t = seq >> 1, s = seq & 1u;
     SubTri st[2];
     Setup S;
@@ -677,14 +690,33 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
         A.t[k] = f2{tc.x * (float)tw, tc.y * (float)th};
     }
     ScanShade SS = prepare_scanline(A, R.lower, R.lr, sp.pl, sp.pr);
-    uint32_t color = bilinear(bm, tw, th, SS.t_left + SS.t_dir * u, oob);
-    f3 center = SS.v + SS.vdir * u;
-    f3 normal = normalize(SS.n + SS.ndir * u);
-    int32_t light = phong(center, normal, J.V.cam, J.ambient, J.sun, J.sun_I, J.lights, (int)J.n_lights);
-    return combine(color, light);
+    if (!GENERIC) {
+        uint32_t color = bilinear(bm, tw, th, SS.t_left + SS.t_dir * u, oob);
+        f3 center = SS.v + SS.vdir * u;
+        f3 normal = normalize(SS.n + SS.ndir * u);
+        int32_t light = phong(center, normal, J.V.cam, J.ambient, J.sun, J.sun_I, J.lights, (int)J.n_lights);
+        return combine(color, light);
+    }
+    uint32_t color = P.color;   // pixel_shader_t::color (default material for material_id -1)
+    if (tk == 1)
+        color = nearest(bm, (uint32_t)tw, (uint32_t)th, SS.t_left + SS.t_dir * u);
+    else if (tk == 2)
+        color = bilinear(bm, tw, th, SS.t_left + SS.t_dir * u, oob);
+    if (lk == 0)
+        return color;
+    int32_t light;
+    if (lk == 1) {
+        light = flat_light(A.w[0], A.w[1], A.w[2], S.inverted, J.V.cam, J.ambient, J.sun, J.sun_I, J.lights,
+                           (int)J.n_lights);
+    } else {
+        f3 center = SS.v + SS.vdir * u;
+        f3 normal = normalize(SS.n + SS.ndir * u);
+        light = phong(center, normal, J.V.cam, J.ambient, J.sun, J.sun_I, J.lights, (int)J.n_lights);
+    }
+    return combined ? combine(color, light) : (uint32_t)light;
 }
 
-template <int TW, int TH>
+template <int TW, int TH, bool GENERIC>
 __device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *keys, uint32_t *s_cnt)
 {
     const int jidx = find_job<JTILE>(B, g);
@@ -733,7 +765,7 @@ __device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *k
         if (k != SW_KEY_EMPTY) {
             seq = (uint32_t)k;
             z = __uint_as_float((uint32_t)(k >> 32));
-            color = shade_pixel(J, seq, x, y, oob);
+            color = shade_pixel<GENERIC>(J, seq, x, y, oob);
         }
         J.out_color[(int64_t)(y - J.out_oy) * J.out_pitch + (x - J.out_ox)] = color;
         const int64_t zo = (int64_t)(y - J.V.vy) * J.V.vw + (x - J.V.vx);
@@ -770,7 +802,7 @@ __device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *k
 // the queue passes the last tile, so tiles of many jobs share the machine.
 // At most 128 VGPRs, so four waves fit on each SIMD (512 registers a lane);
 // the shading path needs about that many and one more halves the overlap.
-template <int TW, int TH>
+template <int TW, int TH, bool GENERIC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_raster(BatchDev B)
 {
     __shared__ unsigned long long keys[TW * TH];
@@ -812,7 +844,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const uint32_t g = s_tile;
         if (g == 0xFFFFFFFFu)
             break;
-        raster_tile<TW, TH>(B, g, keys, s_cnt);
+        raster_tile<TW, TH, GENERIC>(B, g, keys, s_cnt);
         __syncthreads();   // tile done with keys and s_tile before the next claim
     }
     __syncthreads();
@@ -1004,27 +1036,32 @@ hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s)
         LAUNCH_CHECK(k_bin_fill, dim3(blocks), dim3(256), 0, s, B);
     return hipSuccess;
 }
-hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups, hipStream_t s)
+template <bool G>
+static hipError_t launch_raster_t(const BatchDev &B, int tw, int th, dim3 g, hipStream_t s)
 {
-    if (!B.n_tiles)
-        return hipSuccess;
-    const uint32_t groups = B.n_tiles < max_groups ? B.n_tiles : max_groups;
-    dim3 g(groups), b(256);
+    dim3 b(256);
     if (tw == 8 && th == 8)
-        LAUNCH_CHECK((k_raster<8, 8>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<8, 8, G>), g, b, 0, s, B);
     else if (tw == 16 && th == 16)
-        LAUNCH_CHECK((k_raster<16, 16>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<16, 16, G>), g, b, 0, s, B);
     else if (tw == 32 && th == 16)
-        LAUNCH_CHECK((k_raster<32, 16>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<32, 16, G>), g, b, 0, s, B);
     else if (tw == 32 && th == 32)
-        LAUNCH_CHECK((k_raster<32, 32>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<32, 32, G>), g, b, 0, s, B);
     else if (tw == 64 && th == 32)
-        LAUNCH_CHECK((k_raster<64, 32>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<64, 32, G>), g, b, 0, s, B);
     else if (tw == 64 && th == 64)
-        LAUNCH_CHECK((k_raster<64, 64>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<64, 64, G>), g, b, 0, s, B);
     else
         return hipErrorInvalidValue;
     return hipSuccess;
+}
+hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups, bool generic, hipStream_t s)
+{
+    if (!B.n_tiles)
+        return hipSuccess;
+    const dim3 g(B.n_tiles < max_groups ? B.n_tiles : max_groups);
+    return generic ? launch_raster_t<true>(B, tw, th, g, s) : launch_raster_t<false>(B, tw, th, g, s);
 }
 
 hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStream_t s)

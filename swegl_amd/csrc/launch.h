@@ -10,6 +10,6 @@ hipError_t launch_mark(const BatchDev &B, uint32_t blocks, hipStream_t s);
 hipError_t launch_bin_count(const BatchDev &B, uint32_t blocks, hipStream_t s);
 hipError_t launch_scan(const BatchDev &B, hipStream_t s);
 hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s);
-hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups, hipStream_t s);
+hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups, bool generic, hipStream_t s);
 hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStream_t s);
 }  // namespace sw

@@ -138,6 +138,34 @@ SW_D uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
     return out;
 }
 
+// pixel_shader_texture::shade (pixel_shaders.cpp:275-281): twidth and
+// theight are unsigned there, so (int)t % twidth is the unsigned remainder
+// of the int's bits, always inside the bitmap; column from t.x, row from t.y.
+SW_D uint32_t nearest(const uint32_t *pc, uint32_t tw, uint32_t th, f2 t)
+{
+    const uint32_t x = (uint32_t)cvtt(t.x), y = (uint32_t)cvtt(t.y);
+    const uint32_t u = (tw & (tw - 1)) == 0 ? x & (tw - 1) : x % tw;
+    const uint32_t v = (th & (th - 1)) == 0 ? y & (th - 1) : y % th;
+    return pc[v * tw + u];
+}
+
+// pixel_shader_lights_flat::prepare_for_triangle (pixel_shaders.cpp:33-84),
+// returned by shade as int: the face normal normal_t(cross(e1, e2)) is
+// normalised once, and again when negated for an inverted triangle (unary
+// minus builds a normal_t, points.hpp:171-174); the light is evaluated at
+// the centroid ((v0 + v1) + v2) / 3 exactly as Phong evaluates a pixel.
+SW_D int32_t flat_light(f3 w0, f3 w1, f3 w2, bool inverted, f3 cam, float ambient, f3 sun, float sun_I,
+                        const Light4 *lights, int n_lights)
+{
+    const f3 e1 = w1 - w0, e2 = w2 - w0;
+    f3 nrm = normalize({e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x});
+    if (inverted)
+        nrm = normalize(neg(nrm));
+    const f3 s = (w0 + w1) + w2;
+    const f3 center = {s.x / 3.0f, s.y / 3.0f, s.z / 3.0f};
+    return phong(center, nrm, cam, ambient, sun, sun_I, lights, n_lights);
+}
+
 // pixel_shader_light_and_texture::shade combine (pixel_shaders.hpp:159-178)
 SW_D uint32_t combine(uint32_t color, int32_t light_i)
 {

@@ -211,24 +211,59 @@ class camera_t:
 
 
 # ------------------------------------------------------------- shaders ---
-class pixel_shader_lights_phong:
-    pass
+# The pixel shader plugins of swegl/render/pixel_shaders.hpp.  The GPU cannot
+# run virtual calls, so each type names the kernel variant that computes it.
+class pixel_shader_t:
+    """pixel_shader_t: shades with the material colour (pixel_shaders.hpp:15-29)."""
+
+    def kind(self):
+        return A.SHADER_DEFAULT
 
 
-class pixel_shader_texture_bilinear:
-    pass
+class pixel_shader_lights_flat(pixel_shader_t):
+    """One light value per triangle at its centroid (pixel_shaders.cpp:33-84)."""
+
+    def kind(self):
+        return A.SHADER_LIGHTS_FLAT
 
 
-class pixel_shader_light_and_texture:
+class pixel_shader_lights_phong(pixel_shader_t):
+    """Per-pixel Phong (pixel_shaders.cpp:88-205)."""
+
+    def kind(self):
+        return A.SHADER_LIGHTS_PHONG
+
+
+class pixel_shader_texture(pixel_shader_t):
+    """Nearest texel (pixel_shaders.cpp:209-281)."""
+
+    def kind(self):
+        return A.SHADER_TEXTURE_NEAREST
+
+
+class pixel_shader_texture_bilinear(pixel_shader_t):
+    """Bilinear texel (pixel_shaders.cpp:284-384)."""
+
+    def kind(self):
+        return A.SHADER_TEXTURE_BILINEAR
+
+
+_COMBINED = {("flat", "nearest"): A.SHADER_FLAT_NEAREST, ("flat", "bilinear"): A.SHADER_FLAT_BILINEAR,
+             ("phong", "nearest"): A.SHADER_PHONG_NEAREST, ("phong", "bilinear"): A.SHADER_PHONG_BILINEAR}
+_LIGHT = {pixel_shader_lights_flat: "flat", pixel_shader_lights_phong: "phong"}
+_TEX = {pixel_shader_texture: "nearest", pixel_shader_texture_bilinear: "bilinear"}
+
+
+class pixel_shader_light_and_texture(pixel_shader_t):
     """pixel_shader_light_and_texture<L, T> (pixel_shaders.hpp:121-179)."""
 
     def __init__(self, L=pixel_shader_lights_phong, T=pixel_shader_texture_bilinear):
+        if L not in _LIGHT or T not in _TEX:
+            raise TypeError("pixel_shader_light_and_texture takes a light shader type and a texture shader type")
         self.L, self.T = L, T
 
     def kind(self):
-        if self.L is pixel_shader_lights_phong and self.T is pixel_shader_texture_bilinear:
-            return A.SHADER_PHONG_BILINEAR
-        return 0
+        return _COMBINED[(_LIGHT[self.L], _TEX[self.T])]
 
 
 class post_shader_t:
