@@ -173,6 +173,44 @@ def plain_vs_batched(ctx, A, check, L, pairs, iters):
     return out
 
 
+def bench_cube(ctx, A, check, L, seed, frames):
+    """Workload 'cube': make_cube(1, 0) at seeded Euler angles, 256^2 texture,
+    one sun, 640x480, plain path (one render call per frame, eager)."""
+    import numpy as np
+    from swegl_amd import scenes
+    from swegl_amd.device import DeviceScene, Target
+    from swegl_amd.host import SceneDescHolder
+
+    scene, vps = scenes.cube_scene(seed)
+    vp = vps[0]
+    ds = DeviceScene(ctx, SceneDescHolder(scene))
+    col = Target(ctx, A.TARGET_COLOR, vp.m_screen.w, vp.m_screen.h)
+    z = Target(ctx, A.TARGET_FLOAT, vp.m_w, vp.m_h)
+    d = (A.ViewportDesc * 1)()
+    d[0] = vp.desc(col, z)
+    st = (A.FrameStats * 1)()
+    check(L.swegl_render(ctx.h, ds.h, d, 1, st), "swegl_render")
+    for _ in range(5):
+        check(L.swegl_render(ctx.h, ds.h, d, 1, None), "swegl_render")
+    ctx.sync()
+    a = time.perf_counter()
+    for _ in range(frames):
+        check(L.swegl_render(ctx.h, ds.h, d, 1, None), "swegl_render")
+    ctx.sync()
+    el = time.perf_counter() - a
+    lat = []
+    for _ in range(100):
+        t = time.perf_counter()
+        check(L.swegl_render(ctx.h, ds.h, d, 1, None), "swegl_render")
+        ctx.sync()
+        lat.append((time.perf_counter() - t) * 1e3)
+    ds.close()
+    return {"workload": f"cube 12 triangles, 640x480, tex 256^2, one sun, seed {seed}",
+            "frames_per_s": frames / el, "gfragments_per_s": st[0].fragments * frames / el / 1e9,
+            "fragments_per_frame": int(st[0].fragments),
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}}, (scene, vps)
+
+
 def bench_batch(ctx, A, check, L, count, seed, iters):
     """Workload 'batch': `count` height fields (T log-uniform in [1e3, 2e5],
     square framebuffers of 256..1024, DoF on half)."""
@@ -439,6 +477,7 @@ def main():
         "valu_roofline": valu,
         "device": info,
     }
+    out["cube"], cube = bench_cube(ctx, A, check, L, args.seed, 2000)
     if args.batch:
         if "batch" in tuning:
             ctx.apply_tuning(tuning["batch"])
@@ -452,6 +491,7 @@ def main():
         out["heavy"] = bench_heavy(ctx, A, check, L, args.heavy, args.seed + 3, args.heavy_iters)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, vps, args.cpu_seconds, args.cpu_threads)
+        out["cpu_baseline_cube"] = cpu_baseline(cube[0], cube[1], 4.0, args.cpu_threads)
         out["fp64_reshade"] = fp64_reshade(ctx, A, check, L, ds, scene, vp)
     print(json.dumps(out))
 
