@@ -181,10 +181,15 @@ const char *swegl_last_error(void);
 int  swegl_device_info(swegl_ctx *ctx, int *n_cus, char *arch, int arch_cap,
                        uint64_t *hbm_bytes);
 
-/* Raster tuning: tile edge in pixels (8, 16, 32 or 64) and the bin
- * capacity hint; the autotuner writes these. */
+/* Raster tuning: tile size in pixels (8x8, 16x16, 32x16, 32x32, 64x32 or
+ * 64x64); the autotuner writes it. */
 int  swegl_ctx_set_tile(swegl_ctx *ctx, int tile_w, int tile_h);
 int  swegl_ctx_get_tile(swegl_ctx *ctx, int *tile_w, int *tile_h);
+/* With auto tiles on (the default), a render whose viewports give fewer
+ * tiles than there are persistent raster workgroups uses the next smaller
+ * tile sizes (64x64 > 64x32 > 32x32 > 32x16 > 16x16) until they do, so a
+ * small viewport still spreads over the device; off: always the set tile. */
+int  swegl_ctx_set_tile_auto(swegl_ctx *ctx, int on);
 /* Bin limit: a triangle whose tile rectangle holds more than `tiles` tiles
  * is not binned per tile but listed once per viewport and tested by every
  * tile (default 16; 0 sends every triangle through that list). */

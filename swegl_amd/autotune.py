@@ -78,6 +78,7 @@ def tune(ctx, name, pairs, reps, space):
         check(L.swegl_render_batch_graph(ctx.h, jobs, n, None), "render_batch_graph")
 
     results = []
+    ctx.set_tile_auto(False)   # measure each tile size as given
     for tile, limit, groups in space:
         ctx.set_tile(*tile)
         ctx.set_bin_limit(limit)
@@ -85,6 +86,7 @@ def tune(ctx, name, pairs, reps, space):
         ms = _time(ctx, run, reps)
         results.append({"tile": list(tile), "bin_limit": limit, "raster_groups": groups, "ms": ms})
         print(f"{name}: tile {tile} limit {limit} groups {groups}: {ms:.3f} ms", flush=True)
+    ctx.set_tile_auto(True)
     best = min(results, key=lambda r: r["ms"])
     out = dict(best)
     out["candidates"] = results

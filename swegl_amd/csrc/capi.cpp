@@ -116,8 +116,8 @@ struct Resources {
 
 // A captured launch sequence.  Only the resource addresses and grid sizes
 // are baked in; every pointer the kernels follow lives in the job table,
-// which is rewritten into the graph's pinned buffer and copied by the
-// graph's first node on every replay.
+// which is rewritten into one of the graph's two pinned buffers and copied
+// to the device ahead of every replay on the same stream.
 struct GraphEntry {
     Resources res;
     void *pinned[2] = {nullptr, nullptr};
@@ -127,7 +127,6 @@ struct GraphEntry {
     int next = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t copy_node = nullptr;
     ~GraphEntry()
     {
         if (exec)
@@ -165,6 +164,7 @@ struct swegl_ctx {
     int tw = 32, th = 32;
     uint32_t big_tiles = 16; // bin limit (swegl_ctx_set_bin_limit)
     uint32_t groups_per_cu = 4;   // persistent raster workgroups per CU
+    bool tile_auto = true;        // smaller tiles for waves with few of them (swegl_ctx_set_tile_auto)
     unsigned long long *h_stats = nullptr;   // pinned
     bool prof = false;
     std::vector<ProfRec> pending;
@@ -370,6 +370,14 @@ extern "C" int swegl_ctx_set_tile(swegl_ctx *c, int tw, int th)
             return SWEGL_OK;
         }
     return fail(SWEGL_EINVAL, "unsupported tile %dx%d", tw, th);
+}
+
+extern "C" int swegl_ctx_set_tile_auto(swegl_ctx *c, int on)
+{
+    if (!c)
+        return fail(SWEGL_EINVAL, "null ctx");
+    c->tile_auto = on != 0;
+    return SWEGL_OK;
 }
 
 extern "C" int swegl_ctx_get_tile(swegl_ctx *c, int *tw, int *th)
@@ -842,15 +850,41 @@ struct Wave {
     std::vector<JobDev> jd;
     uint32_t vblk = 0, tblk = 0, dblk = 0, tiles = 0, n_ctr = 0;
     int rmax = 0;             // largest DoF radius (sizes the DoF LDS)
+    int tw = 32, th = 32;     // raster tile of this wave
     bool generic = false;     // some job uses a shader other than Phong + bilinear
     uint64_t cap = 0;
     size_t job_offset = 0;   // byte offset of this wave's table in Resources::jobs
 };
 
+// Tile size of a wave: the context's, or with tile_auto, the next smaller
+// size of 64x64 > 64x32 > 32x32 > 32x16 > 16x16 while the wave's viewports
+// give fewer tiles than there are persistent raster workgroups (a small
+// viewport otherwise leaves most CUs idle).
+static void wave_tile(const swegl_ctx *c, const std::vector<JobIn> &jobs, int &tw, int &th)
+{
+    tw = c->tw;
+    th = c->th;
+    if (!c->tile_auto)
+        return;
+    const uint64_t want = (uint64_t)std::max(1, c->n_cus) * c->groups_per_cu;
+    for (;;) {
+        uint64_t tiles = 0;
+        for (const JobIn &j : jobs)
+            tiles += (uint64_t)((j.vp->w + tw - 1) / tw) * ((j.vp->h + th - 1) / th);
+        if (tiles >= want || (tw <= 16 && th <= 16))
+            return;
+        if (tw > th)
+            tw /= 2;    // 64x32 -> 32x32, 32x16 -> 16x16
+        else
+            th /= 2;    // 64x64 -> 64x32, 32x32 -> 32x16
+    }
+}
+
 static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wave &W)
 {
     const uint32_t nj = (uint32_t)jobs.size();
     W.jd.assign(nj, JobDev{});
+    wave_tile(c, jobs, W.tw, W.th);
     std::map<swegl_scene *, int> scene_uses;
     uint64_t tris_total = 0;
     for (uint32_t j = 0; j < nj; j++) {
@@ -925,10 +959,10 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         J.fdist = v.focal_distance;
         J.fdepth = v.focal_depth;
         J.rmax = v.dof_max_radius;
-        J.tw = c->tw;
-        J.th = c->th;
-        J.tiles_x = (v.w + c->tw - 1) / c->tw;
-        J.tiles_y = (v.h + c->th - 1) / c->th;
+        J.tw = W.tw;
+        J.th = W.th;
+        J.tiles_x = (v.w + W.tw - 1) / W.tw;
+        J.tiles_y = (v.h + W.th - 1) / W.th;
         J.tile_base = W.tiles;
         W.tiles += (uint32_t)(J.tiles_x * J.tiles_y);
         J.vblock0 = W.vblk;
@@ -1003,7 +1037,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     STAGE("bin_count", launch_bin_count(B, W.tblk, st));
     STAGE("scan", launch_scan(B, st));
     STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
-    STAGE("raster", launch_raster(B, c->tw, c->th, groups, W.generic, st));
+    STAGE("raster", launch_raster(B, W.tw, W.th, groups, W.generic, st));
     STAGE("dof", launch_dof(B, W.dblk, W.rmax, st));
 #undef STAGE
     return SWEGL_OK;
@@ -1091,10 +1125,10 @@ static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swe
 
 static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
 {
-    std::string k = std::to_string(c->tw) + "x" + std::to_string(c->th) + "/" + std::to_string(c->big_tiles) +
-                    "/" + std::to_string(c->groups_per_cu);
+    std::string k = std::to_string(c->big_tiles) + "/" + std::to_string(c->groups_per_cu);
     for (const Wave &W : waves) {
-        k += "|" + std::to_string(W.jd.size()) + ":" + std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
+        k += "|" + std::to_string(W.tw) + "x" + std::to_string(W.th) + ":" + std::to_string(W.jd.size()) + ":" +
+             std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
              "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap) + "," +
              std::to_string(W.rmax) + (W.generic ? "g" : "");
     }
@@ -1122,9 +1156,7 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
         }
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipMemcpyAsync(g->res.jobs.p, g->pinned[0], g->job_bytes, hipMemcpyHostToDevice,
-                                      c->stream);
-        int lrc = e == hipSuccess ? SWEGL_OK : fail(SWEGL_EIO, "capture copy: %s", hipGetErrorString(e));
+        int lrc = SWEGL_OK;
         for (size_t w = 0; lrc == SWEGL_OK && w < waves.size(); w++)
             lrc = launch_wave(c, c->lane0, g->res, waves[w], w == 0, true, stats != nullptr);
         hipGraph_t graph = nullptr;
@@ -1134,18 +1166,6 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
         if (ee != hipSuccess)
             return fail(SWEGL_EIO, "hipStreamEndCapture: %s", hipGetErrorString(ee));
         g->graph = graph;
-        size_t nn = 0;
-        HIPCHK(hipGraphGetNodes(graph, nullptr, &nn));
-        std::vector<hipGraphNode_t> nodes(nn);
-        HIPCHK(hipGraphGetNodes(graph, nodes.data(), &nn));
-        for (hipGraphNode_t n : nodes) {
-            hipGraphNodeType t;
-            HIPCHK(hipGraphNodeGetType(n, &t));
-            if (t == hipGraphNodeTypeMemcpy)
-                g->copy_node = n;
-        }
-        if (!g->copy_node)
-            return fail(SWEGL_EIO, "captured graph has no job-table copy");
         HIPCHK(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
         G = g.get();
         c->graphs[key] = std::move(g);
@@ -1160,8 +1180,9 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
         HIPCHK(hipEventSynchronize(G->done[slot]));
     for (const Wave &W : waves)
         memcpy((char *)G->pinned[slot] + W.job_offset, W.jd.data(), W.jd.size() * sizeof(JobDev));
-    HIPCHK(hipGraphExecMemcpyNodeSetParams1D(G->exec, G->copy_node, G->res.jobs.p, G->pinned[slot],
-                                             G->job_bytes, hipMemcpyHostToDevice));
+    // the job table goes in by a plain copy ahead of the replay on the same
+    // stream: cheaper on the host than rewriting a copy node of the graph
+    HIPCHK(hipMemcpyAsync(G->res.jobs.p, G->pinned[slot], G->job_bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipGraphLaunch(G->exec, c->stream));
     HIPCHK(hipEventRecord(G->done[slot], c->stream));
     G->used[slot] = true;

@@ -557,8 +557,10 @@ __device__ __forceinline__ float blur_factor(float z, float fdist, float fdepth,
 // depth (and the winner's sequence) once, coalesced.
 struct TileRect { int x0, y0, x1, y1; };   // [x0,x1) x [y0,y1), absolute pixels
 
-__device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, unsigned long long *keys,
-                             int TW, uint32_t &frags, uint32_t &nans)
+// Rows [wy0, wy1) of the tile only, so that one large triangle can be split
+// over several lanes; every row of the tile belongs to exactly one call.
+__device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, int wy1,
+                             unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans)
 {
     SubTri st[2];
     int ns = tri_subs(J, t, st);
@@ -568,18 +570,18 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, unsigned 
         if (s >= ns || !setup_subtri(st[s], J.V, S))
             continue;
         const uint32_t seq = 2u * t + (uint32_t)s;
-        int y = tr.y0;
-        while (y < tr.y1) {
+        int y = wy0;
+        while (y < wy1) {
             bool lower;
             int yend;
             if (S.has_up && y >= S.up_y0 && y < S.up_y1) {
                 lower = false;
-                yend = min(S.up_y1, tr.y1);
+                yend = min(S.up_y1, wy1);
             } else if (S.has_lo && y >= S.lo_y0 && y < S.lo_y1) {
                 lower = true;
-                yend = min(S.lo_y1, tr.y1);
+                yend = min(S.lo_y1, wy1);
             } else {
-                int nxt = tr.y1;
+                int nxt = wy1;
                 if (S.has_up && S.up_y0 > y && S.up_y1 > S.up_y0)
                     nxt = min(nxt, S.up_y0);
                 if (S.has_lo && S.lo_y0 > y && S.lo_y1 > S.lo_y0)
@@ -735,21 +737,99 @@ __device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *k
 
     uint32_t frags = 0, nans = 0, oob = 0;
     // Replay the tile's binned triangles, then the job's big triangles whose
-    // rectangle touches this tile, one per lane.
+    // rectangle touches this tile.  With more than 256 of them, one lane per
+    // triangle.  With fewer, a triangle's rows in the tile are split into
+    // chunks of about 64 pixels (from its screen bounding box) and the chunks
+    // are dealt to the lanes, so a large triangle does not leave one lane
+    // looping over its pixels while the others idle.
+    __shared__ uint32_t s_tri[256], s_rows[256], s_first[256], s_wsum[4];
     const uint32_t nb = B.tile_count[g * SW_TILE_PAD], off = B.tile_offset[g];
     const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
-    for (uint32_t e = threadIdx.x; e < nb + nbig; e += 256) {
-        uint32_t t;
-        if (e < nb) {
-            t = B.bins[off + e];
-        } else {
-            t = J.big[e - nb];
-            const uint2 r = J.rect[t];
-            if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
-                ty > (int)(r.y >> 16))
-                continue;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    if (nb + nbig > 256) {
+        // enough triangles to fill the lanes: one lane per triangle, all rows
+        for (uint32_t e = threadIdx.x; e < nb + nbig; e += 256) {
+            uint32_t t;
+            if (e < nb) {
+                t = B.bins[off + e];
+            } else {
+                t = J.big[e - nb];
+                const uint2 r = J.rect[t];
+                if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
+                    ty > (int)(r.y >> 16))
+                    continue;
+            }
+            raster_depth(J, t, tr, tr.y0, tr.y1, keys, TW, frags, nans);
         }
-        raster_depth(J, t, tr, keys, TW, frags, nans);
+    } else {
+    for (uint32_t base = 0; base < nb + nbig; base += 256) {
+        const uint32_t e = base + threadIdx.x;
+        uint32_t t = 0xFFFFFFFFu, nch = 0;
+        int ry0 = tr.y0, ry1 = tr.y1;
+        if (e < nb + nbig) {
+            if (e < nb) {
+                t = B.bins[off + e];
+            } else {
+                t = J.big[e - nb];
+                const uint2 r = J.rect[t];
+                if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
+                    ty > (int)(r.y >> 16))
+                    t = 0xFFFFFFFFu;
+            }
+        }
+        if (t != 0xFFFFFFFFu) {
+            // rows [ceil(ymin), ceil(ymax)) are exactly the rows fill_triangle_2 visits for an
+            // unclipped triangle; clipped or non-finite ones keep all rows of the tile
+            const uint32_t i0 = J.tri[3 * t], i1 = J.tri[3 * t + 1], i2 = J.tri[3 * t + 2];
+            const f3 a = frustum_to_screen(J.V, ld3(J.ndc, i0)), b = frustum_to_screen(J.V, ld3(J.ndc, i1)),
+                     c = frustum_to_screen(J.V, ld3(J.ndc, i2));
+            const float big = 1073741824.0f;
+            int w = tr.x1 - tr.x0;
+            if (fabsf(a.y) < big && fabsf(b.y) < big && fabsf(c.y) < big && fabsf(a.x) < big &&
+                fabsf(b.x) < big && fabsf(c.x) < big && fminf(a.z, fminf(b.z, c.z)) >= SW_NEAR) {
+                ry0 = max(tr.y0, cvtt(ceilf(fminf(a.y, fminf(b.y, c.y)))));
+                ry1 = min(tr.y1, cvtt(ceilf(fmaxf(a.y, fmaxf(b.y, c.y)))));
+                const int xa = max(tr.x0, (int)floorf(fminf(a.x, fminf(b.x, c.x))));
+                const int xb = min(tr.x1, (int)ceilf(fmaxf(a.x, fmaxf(b.x, c.x))) + 1);
+                w = max(xb - xa, 1);
+            }
+            const int rows = ry1 - ry0;
+            nch = rows > 0 ? (uint32_t)min(max(rows * w / 64, 1), rows) : 0u;
+        }
+        if (nch == 0)
+            ry0 = ry1 = tr.y0;
+        // exclusive prefix of the chunk counts over the block
+        const uint32_t incl = wave_incl_scan(nch);
+        if (lane == 63)
+            s_wsum[wave] = incl;
+        s_tri[threadIdx.x] = t;
+        s_rows[threadIdx.x] = (uint32_t)(ry0 - tr.y0) | ((uint32_t)(ry1 - tr.y0) << 8) | (nch << 16);
+        __syncthreads();
+        uint32_t before = 0;
+        for (int k = 0; k < wave; k++)
+            before += s_wsum[k];
+        const uint32_t total = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+        s_first[threadIdx.x] = before + incl - nch;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < total; i += 256) {
+            // the entry whose chunks contain item i: last k with s_first[k] <= i and chunks
+            int lo = 0, hi = 255;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_first[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            // s_first is non-decreasing and i < total, so entry lo has a chunk holding i
+            // (an entry without chunks shares its s_first with the next entry)
+            const uint32_t pk = s_rows[lo], n = pk >> 16, ch = i - s_first[lo];
+            const int r0 = tr.y0 + (int)(pk & 0xFF), rows = (int)((pk >> 8) & 0xFF) - (int)(pk & 0xFF);
+            const int ya = r0 + (int)((uint32_t)rows * ch / n), yb = r0 + (int)((uint32_t)rows * (ch + 1) / n);
+            raster_depth(J, s_tri[lo], tr, ya, yb, keys, TW, frags, nans);
+        }
+        __syncthreads();   // the LDS tables are rewritten by the next 256 entries
+    }
     }
     __syncthreads();
 

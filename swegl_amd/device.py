@@ -40,6 +40,7 @@ def lib():
             "swegl_ctx_set_profiling": (C.c_int, [vp, C.c_int]),
             "swegl_ctx_set_bin_limit": (C.c_int, [vp, C.c_uint32]),
             "swegl_ctx_set_raster_groups": (C.c_int, [vp, C.c_uint32]),
+            "swegl_ctx_set_tile_auto": (C.c_int, [vp, C.c_int]),
             "swegl_ctx_kernel_times": (C.c_int, [vp, C.c_char_p, C.c_int, C.POINTER(C.c_double),
                                                  C.POINTER(C.c_uint64), C.c_int,
                                                  C.POINTER(C.c_int)]),
@@ -143,6 +144,9 @@ class Context:
 
     def set_raster_groups(self, per_cu):
         check(lib().swegl_ctx_set_raster_groups(self.h, int(per_cu)), "swegl_ctx_set_raster_groups")
+
+    def set_tile_auto(self, on):
+        check(lib().swegl_ctx_set_tile_auto(self.h, int(on)), "swegl_ctx_set_tile_auto")
 
     def apply_tuning(self, t):
         """Apply one entry of swegl_amd/tuning.json: {"tile": [w, h],

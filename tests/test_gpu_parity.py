@@ -109,11 +109,24 @@ def test_bin_limit_paths_agree(ctx, limit):
 
 @pytest.mark.parametrize("tile", [(8, 8), (16, 16), (32, 16), (64, 32), (64, 64)])
 def test_tile_sizes_agree(ctx, tile):
+    """Each tile size as set (automatic tile shrinking off)."""
     scene, vps = scenes.heightfield_scene(6, n=150, width=333, height=211, tex=128, dof=True)
     want = oracle.render(scene, vps)[0]
     ctx.set_tile(*tile)
+    ctx.set_tile_auto(False)
     try:
         got = gpu_render(ctx, scene, vps)[0]
     finally:
         ctx.set_tile(32, 32)
+        ctx.set_tile_auto(True)
     compare(got, want, colour_tol=2)
+
+
+def test_large_triangles_split_over_lanes(ctx):
+    """Few large triangles per tile (a 24x24-vertex grid over 640x480): the
+    depth replay deals row chunks of each triangle to different lanes."""
+    scene, vps = scenes.heightfield_scene(13, n=24, width=640, height=480, tex=256, dof=False)
+    want = oracle.render(scene, vps)[0]
+    got = gpu_render(ctx, scene, vps)[0]
+    compare(got, want)
+    assert got.stats["fragments"] == want.stats["fragments"]
