@@ -78,11 +78,21 @@ SW_D f3 normalize(f3 a)
     }
     return a;
 }
-// cross() builds a normal_t, which normalises (points.cpp:32-37).  Only the
-// z component is ever read on the path.
-SW_D float cross_nz(f3 l, f3 r)
+// cross() builds a normal_t, which normalises (points.cpp:32-37); only the
+// sign of the normalised z component is ever read on the path:
+// sign(normalize(cross(l, r)).z) compared with 0 (dir = +1: > 0, -1: < 0)
+// without the square root and division where they cannot change it: with
+// s = x^2 + y^2 + z^2 (the normalisation's own sum) zero, normalize leaves the
+// vector unchanged; with s <= 2^120 and |z| >= 2^-60, z / sqrt(s) is a
+// normal float of z's sign.  Otherwise (overflow, NaN, tiny z) it divides.
+SW_HD bool cross_z_sign(f3 l, f3 r, int dir)
 {
-    return normalize({l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x}).z;
+    const float x = l.y * r.z - l.z * r.y, y = l.z * r.x - l.x * r.z, z = l.x * r.y - l.y * r.x;
+    const float s = x * x + y * y + z * z;
+    float v = z;
+    if (!(s == 0.0f || (s <= 0x1p120f && fabsf(z) >= 0x1p-60f)))
+        v = z / sqrtf(s);
+    return dir > 0 ? v > 0.0f : v < 0.0f;
 }
 
 // 3x4 affine part of a matrix44_t, row-major.
@@ -221,8 +231,10 @@ struct Setup {
 };
 
 // fill_triangle_2 setup (renderer.cpp:361-460).  Returns false when the
-// sub-triangle spans no scanline (y0 == y2).
-SW_D bool setup_subtri(const SubTri &t, const ViewXf &V, Setup &S)
+// sub-triangle spans no scanline (y0 == y2).  With `row` given, only the
+// short side of the half that processes that row is set up (the other
+// half's row range is still computed); row_state then works for that row.
+SW_D bool setup_subtri(const SubTri &t, const ViewXf &V, Setup &S, int row = INT32_MIN)
 {
     SVert a = t.v[0], b = t.v[1], c = t.v[2];
     S.inverted = !t.front;
@@ -248,28 +260,32 @@ SW_D bool setup_subtri(const SubTri &t, const ViewXf &V, Setup &S)
     S.has_up = S.iy1 >= vy;
     S.up_rows = 0;
     if (S.has_up) {
-        S.Su.ratio = (p1.x - p0.x) / (p1.y - p0.y);
-        S.Su.it.init(p1.y - p0.y, p0.z, p1.z);
         int y = S.iy0 > vy ? S.iy0 : vy;
         int ye = S.iy1 < vy + vh ? S.iy1 : vy + vh;
-        S.Su.it.displace((float)y - p0.y);
-        S.Su.x = p0.x + S.Su.ratio * ((float)y - p0.y);
-        S.lr_up = S.L.ratio > S.Su.ratio;
         S.up_y0 = y;
         S.up_y1 = ye;
         S.up_rows = ye > y ? ye - y : 0;
+        if (row == INT32_MIN || (row >= y && row < ye)) {
+            S.Su.ratio = (p1.x - p0.x) / (p1.y - p0.y);
+            S.Su.it.init(p1.y - p0.y, p0.z, p1.z);
+            S.Su.it.displace((float)y - p0.y);
+            S.Su.x = p0.x + S.Su.ratio * ((float)y - p0.y);
+            S.lr_up = S.L.ratio > S.Su.ratio;
+        }
     }
     S.has_lo = S.iy1 < vy + vh;
     if (S.has_lo) {
-        S.Sl.ratio = (p2.x - p1.x) / (p2.y - p1.y);
-        S.Sl.it.init(p2.y - p1.y, p1.z, p2.z);
         int y = S.iy1 > vy ? S.iy1 : vy;
         int ye = S.iy2 < vy + vh ? S.iy2 : vy + vh;
-        S.Sl.it.displace((float)y - p1.y);
-        S.Sl.x = p1.x + S.Sl.ratio * ((float)y - p1.y);
-        S.lr_lo = S.L.ratio < S.Sl.ratio;
         S.lo_y0 = y;
         S.lo_y1 = ye;
+        if (row == INT32_MIN || (row >= y && row < ye)) {
+            S.Sl.ratio = (p2.x - p1.x) / (p2.y - p1.y);
+            S.Sl.it.init(p2.y - p1.y, p1.z, p2.z);
+            S.Sl.it.displace((float)y - p1.y);
+            S.Sl.x = p1.x + S.Sl.ratio * ((float)y - p1.y);
+            S.lr_lo = S.L.ratio < S.Sl.ratio;
+        }
     }
     return true;
 }

@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_mark(BatchDev B)
     f3 a = ld3(J.ndc, i0), b = ld3(J.ndc, i1), c = ld3(J.ndc, i2);
     if (!inside_frustum(a, b, c))
         return;
-    if (!J.prims[prim_of_tri(J, t)].double_sided && !(cross_nz(b - a, c - a) > 0))
+    if (!J.prims[prim_of_tri(J, t)].double_sided && !cross_z_sign(b - a, c - a, 1))
         return;
     J.yes[i0] = 1;
     J.yes[i1] = 1;
@@ -209,7 +209,7 @@ __device__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
     if (!(J.yes[i0] && J.yes[i1] && J.yes[i2]))
         return 0;
     SVert v0 = orig_vertex(J, i0), v1 = orig_vertex(J, i1), v2 = orig_vertex(J, i2);
-    bool front = cross_nz(v1.s - v0.s, v2.s - v0.s) < 0;
+    bool front = cross_z_sign(v1.s - v0.s, v2.s - v0.s, -1);
     bool inv = false;
     if (v1.s.z > v0.s.z) { SVert x = v0; v0 = v1; v1 = x; inv = !inv; }
     if (v2.s.z > v1.s.z) { SVert x = v1; v1 = v2; v2 = x; inv = !inv; }
@@ -226,7 +226,7 @@ __device__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
         SVert n1 = clip_vertex(J, v0.a, v1.a, cut1);
         float cut2 = (v0.s.z - 0.001f) / (v0.s.z - v2.s.z);
         SVert n2 = clip_vertex(J, v0.a, v2.a, cut2);
-        bool f = cross_nz(n1.s - v0.s, n2.s - v0.s) < 0;
+        bool f = cross_z_sign(n1.s - v0.s, n2.s - v0.s, -1);
         out[0].v[0] = v0;
         out[0].v[1] = n1;
         out[0].v[2] = n2;
@@ -238,12 +238,12 @@ __device__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
         SVert n1 = clip_vertex(J, v0.a, v2.a, cut0);
         float cut1 = (v1.s.z - 0.001f) / (v1.s.z - v2.s.z);
         SVert n2 = clip_vertex(J, v1.a, v2.a, cut1);
-        bool f0 = cross_nz(v1.s - v0.s, n2.s - v0.s) < 0;
+        bool f0 = cross_z_sign(v1.s - v0.s, n2.s - v0.s, -1);
         out[0].v[0] = v0;
         out[0].v[1] = v1;
         out[0].v[2] = n2;
         out[0].front = inv ? !f0 : f0;
-        bool f1 = cross_nz(n2.s - v0.s, n1.s - v0.s) < 0;
+        bool f1 = cross_z_sign(n2.s - v0.s, n1.s - v0.s, -1);
         out[1].v[0] = v0;
         out[1].v[1] = n2;
         out[1].v[2] = n1;
@@ -663,7 +663,7 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
     Span sp;
     // The winner was produced by this very replay, so every step below
     // succeeds; the guards only keep a corrupted key from looping.
-    if ((int)s >= tri_subs(J, t, st) || !setup_subtri(s ? st[1] : st[0], J.V, S) || !row_state(S, y, R) ||
+    if ((int)s >= tri_subs(J, t, st) || !setup_subtri(s ? st[1] : st[0], J.V, S, y) || !row_state(S, y, R) ||
         !span_of(R, J.V, sp) || x < sp.x1 || x >= sp.x2)
         return 0u;
     Interp q = sp.q;
