@@ -138,6 +138,9 @@ typedef struct swegl_scene_desc {
 #define SWEGL_SHADER_LIGHTS_PHONG     9   /* pixel_shader_lights_phong alone                   */
 
 /* post shaders (swegl/render/post_shaders.hpp) */
+/* transparency layers per viewport (viewport_t's transparency_layer_count);
+ * with layers the post shader must be post_shader_t */
+#define SWEGL_MAX_TRANSPARENCY_LAYERS 4
 #define SWEGL_POST_NULL        0   /* post_shader_t, no transparency: no-op (:33-47)   */
 #define SWEGL_POST_DEPTH_BOX   1   /* post_shader_depth_box: box taps (:51-131)          */
 #define SWEGL_POST_DEPTH_TAP9  2   /* same blur radius, 3x3 taps at {-r,0,r}^2           */

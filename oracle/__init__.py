@@ -24,7 +24,8 @@ class OracleViewport(C.Structure):
                 ("view", C.c_float * 16), ("projection", C.c_float * 16),
                 ("camera_position", C.c_float * 3), ("post_shader", C.c_int32),
                 ("focal_distance", C.c_float), ("focal_depth", C.c_float),
-                ("dof_max_radius", C.c_int32), ("color64", A.U32P), ("pixel_shader", C.c_int32)]
+                ("dof_max_radius", C.c_int32), ("color64", A.U32P), ("pixel_shader", C.c_int32),
+                ("transparency_layer_count", C.c_int32)]
 
 
 class OracleStats(C.Structure):
@@ -108,6 +109,7 @@ def render(scene, viewports, surfaces=None, reshade64=False):
         o.projection[:] = [float(v) for v in vp.m_camera.m_projectionmatrix.reshape(16)]
         o.camera_position[:] = [float(v) for v in vp.m_camera.m_center]
         o.pixel_shader = vp.m_pixel_shader.kind()
+        o.transparency_layer_count = vp.transparency_layer_count
         ps = vp.m_post_shader
         o.post_shader = ps.kind if ps is not None else -1
         if ps is not None and ps.kind != A.POST_NULL:

@@ -34,6 +34,8 @@ typedef struct oracle_viewport {
                                        /* post shader; 0 where nothing drew   */
                                        /* (SWEGL_SHADER_PHONG_BILINEAR only)  */
     int32_t   pixel_shader;            /* SWEGL_SHADER_*                      */
+    int32_t   transparency_layer_count;/* 0..SWEGL_MAX_TRANSPARENCY_LAYERS;    */
+                                       /* > 0 only with SWEGL_POST_NULL       */
 } oracle_viewport;
 
 typedef struct oracle_stats {

@@ -212,6 +212,24 @@ struct line_side {
     float x;
 };
 
+/* blend(back, front) (colors.cpp:39-57): alpha from the front, integer
+ * alpha union, channels mixed in double and truncated to 8 bits */
+uint32_t blend(uint32_t back, uint32_t front)
+{
+    const int a = (int)(front >> 24), ba = (int)(back >> 24);
+    const uint32_t na = (uint32_t)(255 - ((255 - a) * (255 - ba) / 255));
+    if (a == 255)
+        return (front & 0xFFFFFFu) | (na << 24);
+    if (a == 0)
+        return (back & 0xFFFFFFu) | (na << 24);
+    uint32_t out = na << 24;
+    for (int c = 0; c < 3; c++) {
+        const double v = ((back >> (8 * c)) & 0xFF) * ((256 - a) / 256.0) + ((front >> (8 * c)) & 0xFF) * (a / 256.0);
+        out |= (uint32_t)to_u8_d(v) << (8 * c);
+    }
+    return out;
+}
+
 /* ---------------- viewport ---------------------------------------------- */
 struct viewport_m {
     int x, y, w, h;
@@ -220,6 +238,11 @@ struct viewport_m {
     float *zb;
     uint32_t *tid;
     uint32_t *color64 = nullptr;   /* w*h FP64 re-shade, or NULL */
+    /* transparency layers (viewport.hpp:27-62): K of them, each a z-buffer
+     * and a colour plane of w*h; layer 0 is the farthest */
+    int K = 0;
+    std::vector<float> lz;
+    std::vector<uint32_t> lc;
     const float *view, *proj;
     vec3 cam;
     float m00, m03, m11, m13;   /* viewport matrix (viewport.cpp:30-35) */
@@ -700,12 +723,65 @@ struct render_sink {
                 nan_fragments++;
             if (z >= vp->zb[off])
                 continue;
-            *video = sh->shade(q.progress());
+            uint32_t color = sh->shade(q.progress());
+            if (vp->K > 0 && (color >> 24) != 255) {
+                insert_layer(off, z, color);   /* transparent: no base write */
+                continue;
+            }
+            *video = color;
             if (sh64)
                 vp->color64[off] = sh64->shade(q.progress());
             vp->zb[off] = z;
             if (vp->tid)
                 vp->tid[off] = seq;
+            if (vp->K > 0)
+                drop_layers_behind(off, z);
+        }
+    }
+    /* renderer.cpp:500-515: the first layer that is empty or nearer than z */
+    int layer_at(int64_t off, float z) const
+    {
+        const int64_t n = (int64_t)vp->w * vp->h;
+        int k = 0;
+        for (; k < vp->K; k++) {
+            float lz = vp->lz[k * n + off];
+            if (lz == max_z() || lz < z)
+                break;
+        }
+        return k;
+    }
+    /* renderer.cpp:516-532: an opaque fragment hides the layers behind it */
+    void drop_layers_behind(int64_t off, float z)
+    {
+        const int64_t n = (int64_t)vp->w * vp->h;
+        int i = 0, k = layer_at(off, z);
+        for (; k < vp->K; i++, k++) {
+            vp->lz[i * n + off] = vp->lz[k * n + off];
+            vp->lc[i * n + off] = vp->lc[k * n + off];
+        }
+        for (; i < vp->K; i++) {
+            vp->lz[i * n + off] = max_z();
+            vp->lc[i * n + off] = 0;
+        }
+    }
+    /* renderer.cpp:533-550: insert a transparent fragment, dropping the
+     * farthest layer when all are used */
+    void insert_layer(int64_t off, float z, uint32_t color)
+    {
+        const int64_t n = (int64_t)vp->w * vp->h;
+        int k = layer_at(off, z);
+        if (vp->lz[(int64_t)(vp->K - 1) * n + off] != max_z()) {
+            while (k-- > 0) {
+                std::swap(vp->lz[k * n + off], z);
+                std::swap(vp->lc[k * n + off], color);
+            }
+        } else {
+            for (; k < vp->K; k++) {
+                std::swap(vp->lz[k * n + off], z);
+                std::swap(vp->lc[k * n + off], color);
+                if (z == max_z())
+                    break;
+            }
         }
     }
     void prepare_for_triangle(uint32_t i0, uint32_t i1, uint32_t i2, bool inv)
@@ -1159,6 +1235,8 @@ void render_viewport(scene_m &sc, viewport_m &vp, const oracle_viewport &ov, ora
         if (vp.color64)
             vp.color64[i] = 0;
     }
+    vp.lz.assign((size_t)vp.K * vp.w * vp.h, max_z());   /* viewport.cpp:106-110 */
+    vp.lc.assign((size_t)vp.K * vp.w * vp.h, 0u);
 
     shader sh{};
     sh.sc = &sc;
@@ -1224,6 +1302,26 @@ void render_viewport(scene_m &sc, viewport_m &vp, const oracle_viewport &ov, ora
         }
     }
 
+    if (vp.K > 0) {
+        /* viewport_t::flatten (viewport.cpp:43-86): layers 1..K-1 blended into
+         * layer 0, then layer 0 over the screen, which is read from the
+         * surface's origin, not the viewport's (:61); post_shader_t::shade
+         * copies layer 0 back to rows y + j but columns from 0 (post_shaders.hpp:22-30) */
+        const int64_t n = (int64_t)vp.w * vp.h;
+        for (int k = 1; k < vp.K; k++)
+            for (int64_t p = 0; p < n; p++)
+                if (vp.lc[k * n + p] >> 24)
+                    vp.lc[p] = blend(vp.lc[p], vp.lc[k * n + p]);
+        for (int j = 0; j < vp.h; j++)
+            for (int i = 0; i < vp.w; i++) {
+                uint32_t &front = vp.lc[(int64_t)j * vp.w + i];
+                const uint32_t back = vp.pixels[(int64_t)j * vp.pitch_words + i];
+                front = (front >> 24) ? blend(back, front) : back;
+            }
+        for (int j = 0; j < vp.h; j++)
+            for (int i = 0; i < vp.w; i++)
+                vp.pixels[(int64_t)(j + vp.y) * vp.pitch_words + i] = vp.lc[(int64_t)j * vp.w + i];
+    }
     if (ov.post_shader == SWEGL_POST_DEPTH_BOX || ov.post_shader == SWEGL_POST_DEPTH_TAP9) {
         std::vector<uint32_t> src((size_t)vp.w * vp.h);
         for (int j = 0; j < vp.h; j++)
@@ -1263,7 +1361,10 @@ int oracle_render(const swegl_scene_desc *scene, oracle_viewport *vps, uint32_t 
             return SWEGL_ENOTSUP;
         if (o.pixel_shader < SWEGL_SHADER_PHONG_BILINEAR || o.pixel_shader > SWEGL_SHADER_LIGHTS_PHONG)
             return SWEGL_ENOTSUP;
-        if (o.color64 && o.pixel_shader != SWEGL_SHADER_PHONG_BILINEAR)
+        if (o.color64 && (o.pixel_shader != SWEGL_SHADER_PHONG_BILINEAR || o.transparency_layer_count))
+            return SWEGL_ENOTSUP;
+        if (o.transparency_layer_count < 0 || o.transparency_layer_count > SWEGL_MAX_TRANSPARENCY_LAYERS ||
+            (o.transparency_layer_count > 0 && o.post_shader != SWEGL_POST_NULL))
             return SWEGL_ENOTSUP;
         const bool textured = o.pixel_shader <= SWEGL_SHADER_FLAT_NEAREST ||
                               o.pixel_shader == SWEGL_SHADER_TEXTURE_NEAREST ||
@@ -1288,6 +1389,7 @@ int oracle_render(const swegl_scene_desc *scene, oracle_viewport *vps, uint32_t 
         vp.zb = o.zbuffer;
         vp.tid = o.triangle_id;
         vp.color64 = o.color64;
+        vp.K = o.transparency_layer_count;
         vp.view = o.view;
         vp.proj = o.projection;
         vp.cam = {o.camera_position[0], o.camera_position[1], o.camera_position[2]};

@@ -22,6 +22,7 @@ SHADER_TEXTURE_NEAREST = 6
 SHADER_TEXTURE_BILINEAR = 7
 SHADER_LIGHTS_FLAT = 8
 SHADER_LIGHTS_PHONG = 9
+MAX_TRANSPARENCY_LAYERS = 4
 POST_NULL, POST_DEPTH_BOX, POST_DEPTH_TAP9 = 0, 1, 2
 
 TARGET_COLOR, TARGET_FLOAT, TARGET_UINT = 0, 1, 2

@@ -78,7 +78,7 @@ struct swegl_target {
 };
 
 struct ViewScratch {
-    DevBuf ndc, yes, rect, big, dof;
+    DevBuf ndc, yes, rect, big, dof, tflat;
 };
 
 struct swegl_scene {
@@ -787,7 +787,7 @@ extern "C" int swegl_scene_bytes(swegl_ctx *c, swegl_scene *s, uint64_t *bytes)
         return fail(SWEGL_EINVAL, "null argument");
     uint64_t b = s->resident;
     for (auto &v : s->views)
-        b += v->ndc.n + v->yes.n + v->rect.n + v->big.n + v->dof.n;
+        b += v->ndc.n + v->yes.n + v->rect.n + v->big.n + v->dof.n + v->tflat.n;
     *bytes = b;
     return SWEGL_OK;
 }
@@ -812,8 +812,12 @@ static int check_viewport(const swegl_viewport_desc &v)
     if (v.triangle_id && (v.triangle_id->kind != SWEGL_TARGET_UINT ||
                           v.triangle_id->w * v.triangle_id->h != v.w * v.h))
         return fail(SWEGL_EINVAL, "triangle id target must hold w*h uints");
-    if (v.transparency_layer_count > 0)
-        return fail(SWEGL_ENOTSUP, "transparency layers are not on the GPU path");
+    static_assert(SW_MAX_LAYERS == SWEGL_MAX_TRANSPARENCY_LAYERS, "layer limit");
+    if (v.transparency_layer_count < 0 || v.transparency_layer_count > SWEGL_MAX_TRANSPARENCY_LAYERS)
+        return fail(SWEGL_ENOTSUP, "transparency_layer_count %d outside [0,%d]", v.transparency_layer_count,
+                    SWEGL_MAX_TRANSPARENCY_LAYERS);
+    if (v.transparency_layer_count > 0 && v.post_shader != SWEGL_POST_NULL)
+        return fail(SWEGL_ENOTSUP, "transparency layers with a depth-of-field post shader");
     if (v.pixel_shader < SWEGL_SHADER_PHONG_BILINEAR || v.pixel_shader > SWEGL_SHADER_LIGHTS_PHONG)
         return fail(SWEGL_ENOTSUP, "pixel shader %d is not a swegl shader type", v.pixel_shader);
     if (v.post_shader < SWEGL_POST_NULL || v.post_shader > SWEGL_POST_DEPTH_TAP9)
@@ -852,6 +856,7 @@ struct Wave {
     int rmax = 0;             // largest DoF radius (sizes the DoF LDS)
     int tw = 32, th = 32;     // raster tile of this wave
     bool generic = false;     // some job uses a shader other than Phong + bilinear
+    bool layers = false;      // the wave's (single) job has transparency layers
     uint64_t cap = 0;
     size_t job_offset = 0;   // byte offset of this wave's table in Resources::jobs
 };
@@ -864,6 +869,11 @@ static void wave_tile(const swegl_ctx *c, const std::vector<JobIn> &jobs, int &t
 {
     tw = c->tw;
     th = c->th;
+    for (const JobIn &j : jobs)
+        if (j.vp->transparency_layer_count > 0) {
+            tw = th = 16;   // the in-order layer raster works on 16x16 tiles
+            return;
+        }
     if (!c->tile_auto)
         return;
     const uint64_t want = (uint64_t)std::max(1, c->n_cus) * c->groups_per_cu;
@@ -897,6 +907,8 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
             (rc = vs.big.ensure(T * 4)))
             return rc;
         const bool dof = v.post_shader != SWEGL_POST_NULL;
+        if (v.transparency_layer_count > 0 && (rc = vs.tflat.ensure((size_t)v.w * v.h * 4)))
+            return rc;
         if (dof && (rc = vs.dof.ensure((size_t)v.w * v.h * 5)))   // colour, then one code byte per pixel
             return rc;
         JobDev &J = W.jd[j];
@@ -954,6 +966,9 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         J.zbuf = v.depth->buf.as<float>();
         J.tid = v.triangle_id ? v.triangle_id->buf.as<uint32_t>() : nullptr;
         J.shader = v.pixel_shader;
+        J.layers = v.transparency_layer_count;
+        J.tflat = v.transparency_layer_count > 0 ? vs.tflat.as<uint32_t>() : nullptr;
+        W.layers = W.layers || v.transparency_layer_count > 0;
         W.generic = W.generic || v.pixel_shader != SWEGL_SHADER_PHONG_BILINEAR;
         J.post = v.post_shader;
         J.fdist = v.focal_distance;
@@ -1034,11 +1049,18 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     } while (0)
     STAGE("vertex", launch_vertex(B, W.vblk, zero0, W.n_ctr - zero0, st));
     STAGE("mark", launch_mark(B, W.tblk, st));
-    STAGE("bin_count", launch_bin_count(B, W.tblk, st));
-    STAGE("scan", launch_scan(B, st));
-    STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
-    STAGE("raster", launch_raster(B, W.tw, W.th, groups, W.generic, st));
-    STAGE("dof", launch_dof(B, W.dblk, W.rmax, st));
+    STAGE("bin_count", launch_bin_count(B, W.tblk, st));   // also writes each triangle's tile rectangle
+    if (W.layers) {
+        // the layer raster walks each tile's triangles in draw order through
+        // their tile rectangles: no bins
+        const uint32_t pixels = (uint32_t)(W.jd[0].V.vw * W.jd[0].V.vh);
+        STAGE("raster", launch_layers(B, W.generic, pixels, st));
+    } else {
+        STAGE("scan", launch_scan(B, st));
+        STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
+        STAGE("raster", launch_raster(B, W.tw, W.th, groups, W.generic, st));
+        STAGE("dof", launch_dof(B, W.dblk, W.rmax, st));
+    }
 #undef STAGE
     return SWEGL_OK;
 }
@@ -1066,8 +1088,9 @@ static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::ve
     for (const JobIn &j : all) {
         bool clash = false;
         for (const JobIn &k : groups.back())
-            if (conflicts(*j.vp, *k.vp))
-                clash = true;
+            if (conflicts(*j.vp, *k.vp) || j.vp->transparency_layer_count > 0 ||
+                k.vp->transparency_layer_count > 0)
+                clash = true;   // a job with layers reads and writes outside its viewport: alone
         if (clash)
             groups.emplace_back();
         groups.back().push_back(j);
@@ -1130,7 +1153,7 @@ static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
         k += "|" + std::to_string(W.tw) + "x" + std::to_string(W.th) + ":" + std::to_string(W.jd.size()) + ":" +
              std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
              "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap) + "," +
-             std::to_string(W.rmax) + (W.generic ? "g" : "");
+             std::to_string(W.rmax) + (W.generic ? "g" : "") + (W.layers ? "L" : "");
     }
     return k;
 }

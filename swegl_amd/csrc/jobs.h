@@ -44,6 +44,7 @@ struct TexDev {
 };
 
 #define SW_MAX_DOF_RADIUS 16
+#define SW_MAX_LAYERS 4   // transparency layers per viewport (SWEGL_MAX_TRANSPARENCY_LAYERS)
 // Depth-of-field output tile (pixels) of one 256-thread workgroup.
 #define SW_DOF_TW 64
 #define SW_DOF_TH 16
@@ -81,6 +82,8 @@ struct JobDev {
     int32_t shader;           // SWEGL_SHADER_*
     int32_t post;             // SWEGL_POST_*
     uint8_t *dofc;            // DoF: per pixel (blur factor != 0) | radius << 1, w*h
+    int32_t layers;           // transparency layers (0: none)
+    uint32_t *tflat;          // with layers: flattened layer colour per pixel, w*h
     float fdist, fdepth;
     int32_t rmax;
     // ---- tiles

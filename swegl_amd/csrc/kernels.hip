@@ -938,6 +938,241 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 }
 
+// ------------------------------------------------------- k_raster_layers
+// Viewports with transparency layers (renderer.cpp:486-550).  The layer
+// updates depend on draw order, so each 16x16 tile walks the job's
+// triangles in draw order (those whose tile rectangle holds the tile, 256 at
+// a time, compacted in order; k_bin_count wrote the rectangles) and every lane owns one pixel: its base depth,
+// colour and sequence and its K layers live in registers, and the lane
+// applies each fragment of its pixel in order -- z test against the base,
+// shade, then the base write (opaque, alpha 255) with the farther layers
+// dropped, or the layer insertion (transparent).  Fragments of one triangle
+// fall on distinct pixels, so lanes never share state.  At the end the lane
+// writes the base colour, depth and sequence and the layers flattened into
+// layer 0 (viewport.cpp:80-84); k_tflatten/k_tcopy finish the flatten and
+// post_shader_t's copy (viewport.cpp:53-67, post_shaders.hpp:22-47).
+__device__ __forceinline__ uint32_t blend_colors(uint32_t back, uint32_t front)
+{
+    // colors.cpp:39-57: alpha from the front, integer alpha union, channels
+    // mixed in double and truncated to 8 bits
+    const int a = (int)(front >> 24), ba = (int)(back >> 24);
+    const uint32_t na = (uint32_t)(255 - ((255 - a) * (255 - ba) / 255));
+    if (a == 255)
+        return (front & 0xFFFFFFu) | (na << 24);
+    if (a == 0)
+        return (back & 0xFFFFFFu) | (na << 24);
+    uint32_t out = na << 24;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double v = (double)((back >> (8 * c)) & 0xFF) * ((256 - a) / 256.0) +
+                         (double)((front >> (8 * c)) & 0xFF) * (a / 256.0);
+        out |= (uint32_t)(uint8_t)(int)v << (8 * c);   // v in [0, 255]
+    }
+    return out;
+}
+
+template <bool GENERIC>
+__global__ __launch_bounds__(256) void k_raster_layers(BatchDev B)
+{
+    constexpr int TW = 16, TH = 16, KMAX = SW_MAX_LAYERS;
+    __shared__ uint32_t s_list[256], s_wcount[4];
+    __shared__ uint32_t s_cnt[3];
+    const uint32_t g = blockIdx.x;
+    const int jidx = find_job<JTILE>(B, g);
+    const JobDev &J = B.jobs[jidx];
+    const int lt = (int)(g - J.tile_base);
+    const int tx = lt % J.tiles_x, ty = lt / J.tiles_x;
+    const int x = J.V.vx + tx * TW + (int)(threadIdx.x % TW), y = J.V.vy + ty * TH + (int)(threadIdx.x / TW);
+    const bool mine = x < J.V.vx + J.V.vw && y < J.V.vy + J.V.vh;
+    const int K = J.layers;
+    const float maxz = __uint_as_float(SW_MAX_Z_BITS);
+    float bz = maxz, lz[KMAX];
+    uint32_t bc = 0u, bseq = 0xFFFFFFFFu, lc[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        lz[k] = maxz;
+        lc[k] = 0u;
+    }
+    uint32_t frags = 0, nans = 0, oob = 0;
+    if (threadIdx.x < 3)
+        s_cnt[threadIdx.x] = 0;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (uint32_t base = 0; base < J.n_tris; base += 256) {
+        // the triangles of this batch whose tile rectangle holds the tile, in draw order
+        const uint32_t t = base + threadIdx.x;
+        bool cov = false;
+        if (t < J.n_tris) {
+            const uint2 r = J.rect[t];
+            cov = r.x != RECT_EMPTY && tx >= (int)(r.x & 0xFFFF) && tx <= (int)(r.y & 0xFFFF) &&
+                  ty >= (int)(r.x >> 16) && ty <= (int)(r.y >> 16);
+        }
+        const uint64_t bal = __ballot(cov);
+        if (lane == 0)
+            s_wcount[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int k = 0; k < 4; k++) {
+            before += k < wave ? s_wcount[k] : 0u;
+            total += s_wcount[k];
+        }
+        if (cov)
+            s_list[before + (uint32_t)__popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = t;
+        __syncthreads();
+        for (uint32_t i = 0; i < total && mine; i++) {
+            const uint32_t tt = s_list[i];
+            SubTri st[2];
+            const int ns = tri_subs(J, tt, st);
+            for (int sub = 0; sub < ns; sub++) {
+                Setup S;
+                Row R;
+                Span sp;
+                if (!setup_subtri(st[sub], J.V, S, y) || !row_state(S, y, R) || !span_of(R, J.V, sp) ||
+                    x < sp.x1 || x >= sp.x2)
+                    continue;
+                Interp q = sp.q;
+                q.steps(x - sp.x1);
+                float z = q.value(q.u());
+                if (z < SW_NEAR)
+                    continue;
+                frags++;
+                if (z != z) {
+                    nans++;
+                    continue;
+                }
+                if (z >= bz)
+                    continue;
+                const uint32_t seq = 2u * tt + (uint32_t)sub;
+                uint32_t color = shade_pixel<GENERIC>(J, seq, x, y, oob);
+                // the first layer that is empty or nearer than z (renderer.cpp:500-505)
+                int at = K;
+#pragma unroll
+                for (int k = KMAX - 1; k >= 0; k--)
+                    if (k < K && (lz[k] == maxz || lz[k] < z))
+                        at = k;
+                if ((color >> 24) == 255) {
+                    // opaque: the new base; layers behind it drop out (:506-522)
+                    bz = z;
+                    bc = color;
+                    bseq = seq;
+                    float nz[KMAX];
+                    uint32_t nc[KMAX];
+#pragma unroll
+                    for (int k = 0; k < KMAX; k++) {
+                        const int from = k + at;
+                        nz[k] = maxz;
+                        nc[k] = 0u;
+#pragma unroll
+                        for (int m = 0; m < KMAX; m++)
+                            if (m == from && m < K) {
+                                nz[k] = lz[m];
+                                nc[k] = lc[m];
+                            }
+                    }
+#pragma unroll
+                    for (int k = 0; k < KMAX; k++) {
+                        lz[k] = nz[k];
+                        lc[k] = nc[k];
+                    }
+                } else {
+                    // transparent: insert at `at` (:523-550)
+                    bool all_used = false;
+#pragma unroll
+                    for (int k = 0; k < KMAX; k++)
+                        if (k == K - 1)
+                            all_used = lz[k] != maxz;
+                    float cz = z;
+                    uint32_t cc = color;
+                    if (all_used) {
+                        // shift toward layer 0: the new fragment lands at at-1, layer 0 falls out
+#pragma unroll
+                        for (int k = KMAX - 1; k >= 0; k--)
+                            if (k < at) {
+                                const float tz = lz[k];
+                                const uint32_t tc = lc[k];
+                                lz[k] = cz;
+                                lc[k] = cc;
+                                cz = tz;
+                                cc = tc;
+                            }
+                    } else {
+                        // shift away from layer 0 up to the first empty layer
+                        bool done = false;
+#pragma unroll
+                        for (int k = 0; k < KMAX; k++)
+                            if (!done && k >= at && k < K) {
+                                const float tz = lz[k];
+                                const uint32_t tc = lc[k];
+                                lz[k] = cz;
+                                lc[k] = cc;
+                                cz = tz;
+                                cc = tc;
+                                done = cz == maxz;
+                            }
+                    }
+                }
+            }
+        }
+        __syncthreads();   // s_list is rewritten by the next batch
+    }
+    if (mine) {
+        J.out_color[(int64_t)(y - J.out_oy) * J.out_pitch + (x - J.out_ox)] = bc;
+        const int64_t zo = (int64_t)(y - J.V.vy) * J.V.vw + (x - J.V.vx);
+        J.zbuf[zo] = bz;
+        if (J.tid)
+            J.tid[zo] = bseq;
+        // layers 1..K-1 flattened into layer 0 (viewport.cpp:43-51, 80-82)
+        uint32_t f = lc[0];
+#pragma unroll
+        for (int k = 1; k < KMAX; k++)
+            if (k < K && (lc[k] >> 24) != 0)
+                f = blend_colors(f, lc[k]);
+        J.tflat[zo] = f;
+    }
+    if (B.stats) {
+        if (frags)
+            atomicAdd(&s_cnt[0], frags);
+        if (nans)
+            atomicAdd(&s_cnt[1], nans);
+        if (oob)
+            atomicAdd(&s_cnt[2], oob);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (s_cnt[0])
+                stat_add(B, SW_ST_FRAGMENTS, s_cnt[0]);
+            if (s_cnt[1])
+                stat_add(B, SW_ST_NAN, s_cnt[1]);
+            if (s_cnt[2])
+                stat_add(B, SW_ST_OOB, s_cnt[2]);
+        }
+    }
+}
+
+// viewport_t::flatten(layer 0) (viewport.cpp:53-67): layer 0 over the screen
+// pixel at (i, j) from the surface's origin -- the reference ignores the
+// viewport's offset here -- or the screen pixel where layer 0 is empty.
+__global__ __launch_bounds__(256) void k_tflatten(BatchDev B, uint32_t job)
+{
+    const JobDev &J = B.jobs[job];
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= (uint32_t)(J.V.vw * J.V.vh))
+        return;
+    const int i = (int)(p % (uint32_t)J.V.vw), j = (int)(p / (uint32_t)J.V.vw);
+    const uint32_t back = J.surface[(int64_t)j * J.surface_pitch + i];
+    const uint32_t f = J.tflat[p];
+    J.tflat[p] = (f >> 24) != 0 ? blend_colors(back, f) : back;
+}
+// post_shader_t::shade with layers (post_shaders.hpp:22-47): layer 0 to the
+// screen rows y + j, columns from 0 (the reference ignores the x offset).
+__global__ __launch_bounds__(256) void k_tcopy(BatchDev B, uint32_t job)
+{
+    const JobDev &J = B.jobs[job];
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= (uint32_t)(J.V.vw * J.V.vh))
+        return;
+    const int i = (int)(p % (uint32_t)J.V.vw), j = (int)(p / (uint32_t)J.V.vw);
+    J.surface[(int64_t)(j + J.V.vy) * J.surface_pitch + i] = J.tflat[p];
+}
+
 // ---------------------------------------------------------------- k_dof
 // Depth of field (build spec of post_shader_depth_box, post_shaders.hpp:51-131
 // with remap_clipped, lerp.hpp:24-43): 64x16 output pixels per block, colour
@@ -1144,6 +1379,19 @@ hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups,
     return generic ? launch_raster_t<true>(B, tw, th, g, s) : launch_raster_t<false>(B, tw, th, g, s);
 }
 
+hipError_t launch_layers(const BatchDev &B, bool generic, uint32_t pixels, hipStream_t s)
+{
+    if (!B.n_tiles)
+        return hipSuccess;
+    if (generic)
+        LAUNCH_CHECK(k_raster_layers<true>, dim3(B.n_tiles), dim3(256), 0, s, B);
+    else
+        LAUNCH_CHECK(k_raster_layers<false>, dim3(B.n_tiles), dim3(256), 0, s, B);
+    const uint32_t blocks = (pixels + 255) / 256;
+    LAUNCH_CHECK(k_tflatten, dim3(blocks), dim3(256), 0, s, B, 0u);
+    LAUNCH_CHECK(k_tcopy, dim3(blocks), dim3(256), 0, s, B, 0u);
+    return hipSuccess;
+}
 hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStream_t s)
 {
     if (blocks)

@@ -11,5 +11,8 @@ hipError_t launch_bin_count(const BatchDev &B, uint32_t blocks, hipStream_t s);
 hipError_t launch_scan(const BatchDev &B, hipStream_t s);
 hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s);
 hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups, bool generic, hipStream_t s);
+// A wave of one job with transparency layers (16x16 tiles): the in-order
+// layer raster, then the flatten over the screen and the copy back.
+hipError_t launch_layers(const BatchDev &B, bool generic, uint32_t pixels, hipStream_t s);
 hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStream_t s);
 }  // namespace sw
