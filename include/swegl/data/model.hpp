@@ -6,8 +6,13 @@
 #pragma once
 
 #include <cstdint>
+#include <cstring>
 #include <memory>
+#include <stdexcept>
+#include <string>
 #include <vector>
+
+#include "../../swegl_hip.h"
 
 #include "../projection/points.hpp"
 #include "../render/colors.hpp"
@@ -85,6 +90,28 @@ public:
     }
 };
 
+// animation_step_t, animation_channel_t, animation_t (model.hpp:89-127)
+struct animation_step_t {
+    float time;
+    vec4f_t value;
+};
+struct animation_channel_t {
+    enum path_t {
+        SCALE = SWEGL_PATH_SCALE,
+        ROTATION = SWEGL_PATH_ROTATION,
+        TRANSLATION = SWEGL_PATH_TRANSLATION,
+        WEIGHTS = SWEGL_PATH_NONE,
+        NONE = SWEGL_PATH_NONE,
+    };
+    int node_idx;
+    path_t path;
+    std::vector<animation_step_t> steps;   // sorted by time
+};
+struct animation_t {
+    float end_time = 0.0f;
+    std::vector<animation_channel_t> channels;
+};
+
 struct scene_t {
     std::vector<node_t> nodes;
     std::vector<int> root_nodes;
@@ -95,9 +122,52 @@ struct scene_t {
     std::vector<point_source_light> point_source_lights;
     std::vector<material_t> materials;
     std::vector<texture_t> images;
+    std::vector<animation_t> animations;
     // Bump after editing vertices, indices or textures in place: render()
     // re-uploads meshes and textures only when this changes.
     std::uint64_t device_generation = 0;
+
+    // scene_t::animate (model.hpp:146-177): poses the animated nodes; the
+    // next render picks the poses up.
+    void animate(const float elapsed_seconds)
+    {
+        if (animations.empty())
+            return;
+        std::vector<swegl_node_desc> nd(nodes.size());
+        for (size_t i = 0; i < nodes.size(); i++) {
+            const node_t &n = nodes[i];
+            nd[i].scale[0] = n.scale.x();
+            nd[i].scale[1] = n.scale.y();
+            nd[i].scale[2] = n.scale.z();
+            std::memcpy(nd[i].rotation, &n.rotation.m[0][0], sizeof nd[i].rotation);
+            nd[i].translation[0] = n.translation.x();
+            nd[i].translation[1] = n.translation.y();
+            nd[i].translation[2] = n.translation.z();
+        }
+        std::vector<std::vector<swegl_anim_step>> steps;
+        std::vector<std::vector<swegl_anim_channel>> chans(animations.size());
+        std::vector<swegl_animation> anims(animations.size());
+        for (size_t a = 0; a < animations.size(); a++) {
+            for (const animation_channel_t &c : animations[a].channels) {
+                std::vector<swegl_anim_step> st;
+                for (const animation_step_t &s : c.steps)
+                    st.push_back({s.time, {s.value.x(), s.value.y(), s.value.z(), s.value.w()}});
+                steps.push_back(std::move(st));
+                chans[a].push_back({c.node_idx, (int32_t)c.path, (uint32_t)steps.back().size(),
+                                    steps.back().data()});
+            }
+            anims[a] = {animations[a].end_time, (uint32_t)chans[a].size(), chans[a].data()};
+        }
+        if (swegl_animate(anims.data(), (uint32_t)anims.size(), elapsed_seconds, nd.data(),
+                          (uint32_t)nd.size()) != SWEGL_OK)
+            throw std::runtime_error(std::string("swegl::scene_t::animate: ") + swegl_last_error());
+        for (size_t i = 0; i < nodes.size(); i++) {
+            node_t &n = nodes[i];
+            n.scale = vertex_t(nd[i].scale[0], nd[i].scale[1], nd[i].scale[2]);
+            std::memcpy(&n.rotation.m[0][0], nd[i].rotation, sizeof nd[i].rotation);
+            n.translation = vertex_t(nd[i].translation[0], nd[i].translation[1], nd[i].translation[2]);
+        }
+    }
 };
 
 // make_cube (model.hpp:254-325): six 4-vertex triangle fans.

@@ -84,6 +84,24 @@ public:
     float y() const { return _y; }
 };
 
+// vec4f_t (swegl/projection/vec2f.hpp:45-93): keyframe values of animations
+class vec4f_t
+{
+    float _x = 0, _y = 0, _z = 0, _w = 0;
+
+public:
+    vec4f_t() = default;
+    vec4f_t(float x, float y, float z, float w) : _x(x), _y(y), _z(z), _w(w) {}
+    float &x() { return _x; }
+    float x() const { return _x; }
+    float &y() { return _y; }
+    float y() const { return _y; }
+    float &z() { return _z; }
+    float z() const { return _z; }
+    float &w() { return _w; }
+    float w() const { return _w; }
+};
+
 // 4x4 row-major float matrix (matrix44_t, freon::Matrix<float,4,4> storage).
 class matrix44_t
 {
@@ -103,6 +121,13 @@ public:
     void rotate_y(float a) { swegl_matrix_rotate_y(&m[0][0], a); }
     void rotate_z(float a) { swegl_matrix_rotate_z(&m[0][0], a); }
     void translate(float x, float y, float z) { swegl_matrix_translate(&m[0][0], x, y, z); }
+    // matrix44.hpp:34-43: q0 is the scalar part
+    static matrix44_t from_quaternion(float q0, float q1, float q2, float q3)
+    {
+        matrix44_t r;
+        swegl_matrix_from_quaternion(&r.m[0][0], q0, q1, q2, q3);
+        return r;
+    }
 };
 
 }  // namespace swegl

@@ -2,7 +2,9 @@
 // reference's swegl/swegl.hpp).  Link with -lswegl_hip.
 #pragma once
 
+#include "data/gltf.hpp"
 #include "data/model.hpp"
+#include "misc/image.hpp"
 #include "projection/camera.hpp"
 #include "projection/points.hpp"
 #include "render/colors.hpp"

@@ -46,7 +46,12 @@ typedef struct swegl_server swegl_server;
 
 /* ---- scene description (swegl/data/model.hpp) --------------------------- */
 
-/* primitive_t::index_mode_t (model.hpp:33-42); only triangle modes render */
+/* primitive_t::index_mode_t (model.hpp:33-42); only triangle modes render,
+ * points and lines are accepted and draw nothing, as in the reference */
+#define SWEGL_POINTS          0
+#define SWEGL_LINES           1
+#define SWEGL_LINE_LOOP       2
+#define SWEGL_LINE_STRIP      3
 #define SWEGL_TRIANGLES       4
 #define SWEGL_TRIANGLE_STRIP  5
 #define SWEGL_TRIANGLE_FAN    6
@@ -159,7 +164,7 @@ typedef struct swegl_viewport_desc {
     float         focal_distance;    /* post_shader_depth_box::focal_distance     */
     float         focal_depth;       /* post_shader_depth_box::focal_depth        */
     int32_t       dof_max_radius;    /* 5 in the reference (post_shaders.hpp:69)  */
-    int32_t       transparency_layer_count; /* must be 0 on the GPU path          */
+    int32_t       transparency_layer_count; /* 0..SWEGL_MAX_TRANSPARENCY_LAYERS   */
 } swegl_viewport_desc;
 
 /* Per-render counters, filled when the render's work completes. */
@@ -305,6 +310,59 @@ int  swegl_server_get_stats(swegl_server *srv, swegl_server_stats *st);
 int  swegl_server_run_trace(swegl_server *srv, const swegl_job *jobs, const double *arrival_s,
                             uint32_t n, double *latency_s);
 
+/* ---- assets: glTF 2.0 / GLB files (src/data/gltf.cpp) ------------------ */
+/* A loaded file in host memory: the scene arrays and the animations.  The
+ * reading follows the reference's loader, including where it departs from
+ * the glTF specification (transposed TEXCOORD_0, 16-bit index reads, the
+ * quaternion's x taken as its scalar part, the row-length scale of a node
+ * matrix); PNG and BMP textures are decoded, JPEG returns SWEGL_ENOTSUP. */
+typedef struct swegl_asset swegl_asset;
+
+/* animation_step_t (model.hpp:90-94): value is (x, y, z) for scale and
+ * translation, (x, y, z, w) for rotation */
+typedef struct swegl_anim_step {
+    float time;
+    float value[4];
+} swegl_anim_step;
+/* animation_channel_t::path_t (model.hpp:97-105) */
+#define SWEGL_PATH_SCALE       0
+#define SWEGL_PATH_ROTATION    1
+#define SWEGL_PATH_TRANSLATION 2
+#define SWEGL_PATH_NONE        3   /* "weights" and unknown paths: not applied */
+/* animation_channel_t (model.hpp:95-122): steps sorted by time */
+typedef struct swegl_anim_channel {
+    int32_t                node;
+    int32_t                path;
+    uint32_t               n_steps;
+    const swegl_anim_step *steps;
+} swegl_anim_channel;
+/* animation_t (model.hpp:123-127) */
+typedef struct swegl_animation {
+    float                     end_time;
+    uint32_t                  n_channels;
+    const swegl_anim_channel *channels;
+} swegl_animation;
+
+/* swegl::load_scene(filename) (src/data/gltf.cpp:455-464, .glb or .gltf). */
+int  swegl_asset_load(const char *filename, swegl_asset **out);
+/* The asset as a scene description for swegl_scene_upload; its arrays
+ * belong to the asset.  Lights, ambient and sun are left at zero (sun
+ * direction (0, -1, 0)), as the reference leaves them to the caller. */
+int  swegl_asset_scene(const swegl_asset *asset, swegl_scene_desc *desc);
+/* scene_t::animations (model.hpp:141), owned by the asset */
+int  swegl_asset_animations(const swegl_asset *asset, const swegl_animation **animations,
+                            uint32_t *n_animations);
+void swegl_asset_destroy(swegl_asset *asset);
+/* scene_t::animate(elapsed_seconds) (model.hpp:146-177): sets the targeted
+ * nodes' rotation, translation and scale; pass the result to
+ * swegl_scene_update_pose. */
+int  swegl_animate(const swegl_animation *animations, uint32_t n_animations, float elapsed_seconds,
+                   swegl_node_desc *nodes, uint32_t n_nodes);
+/* read_image_file (src/misc/image.cpp:20-28) for .png and .bmp: row-major
+ * pixel_colors words; free with swegl_image_free. */
+int  swegl_image_read(const char *filename, uint32_t **pixels, uint32_t *width, uint32_t *height);
+void swegl_image_free(uint32_t *pixels);
+
 /* ---- host math helpers (same float semantics as the reference) --------- */
 /* matrix44_t::rotate_x/y/z (src/projection/matrix44.cpp:12-67), translate
  * (:87-92), swegl::scale (src/projection/points.cpp:14-30). */
@@ -312,6 +370,8 @@ void swegl_matrix_rotate_x(float m[16], float a);
 void swegl_matrix_rotate_y(float m[16], float a);
 void swegl_matrix_rotate_z(float m[16], float a);
 void swegl_matrix_translate(float m[16], float x, float y, float z);
+/* matrix44_t::from_quaternion(q0, q1, q2, q3) (swegl/projection/matrix44.hpp:34-43) */
+void swegl_matrix_from_quaternion(float m[16], float q0, float q1, float q2, float q3);
 /* camera_t(aspect) (src/projection/camera.cpp:8-35): view, projection,
  * center. */
 void swegl_camera_init(float aspect_ratio, float view[16], float projection[16],

@@ -1141,9 +1141,10 @@ int import_scene(const swegl_scene_desc *d, scene_m &sc)
             primitive pr;
             pr.mode = pd.mode;
             pr.material_id = pd.material_id;
-            if (pd.mode != SWEGL_TRIANGLES && pd.mode != SWEGL_TRIANGLE_STRIP &&
-                pd.mode != SWEGL_TRIANGLE_FAN)
-                return SWEGL_ENOTSUP;
+            /* points and lines are read but never drawn (renderer.cpp:83-185,
+             * 197-229 only handle the triangle modes) */
+            if (pd.mode < SWEGL_POINTS || pd.mode > SWEGL_TRIANGLE_FAN)
+                return SWEGL_EINVAL;
             if (pd.material_id < -1 || pd.material_id >= (int)d->n_materials)
                 return SWEGL_EINVAL;
             int ti = pd.material_id == -1 ? -1 : d->materials[pd.material_id].texture_idx;

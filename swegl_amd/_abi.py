@@ -10,8 +10,12 @@ EIO, EAGAIN, ENOMEM, ENODEV, EINVAL, ERANGE, ENOTSUP = -5, -11, -12, -19, -22, -
 ERRNAMES = {0: "OK", -5: "EIO", -11: "EAGAIN", -12: "ENOMEM", -19: "ENODEV",
             -22: "EINVAL", -34: "ERANGE", -95: "ENOTSUP"}
 
-# primitive_t::index_mode_t (swegl/data/model.hpp:33-42)
+# primitive_t::index_mode_t (swegl/data/model.hpp:33-42); points and lines draw nothing
+POINTS, LINES, LINE_LOOP, LINE_STRIP = 0, 1, 2, 3
 TRIANGLES, TRIANGLE_STRIP, TRIANGLE_FAN = 4, 5, 6
+
+# animation_channel_t::path_t (swegl/data/model.hpp:97-105)
+PATH_SCALE, PATH_ROTATION, PATH_TRANSLATION, PATH_NONE = 0, 1, 2, 3
 
 SHADER_PHONG_BILINEAR = 1
 SHADER_PHONG_NEAREST = 2
@@ -79,6 +83,20 @@ class ViewportDesc(C.Structure):
                 ("post_shader", C.c_int32), ("focal_distance", C.c_float),
                 ("focal_depth", C.c_float), ("dof_max_radius", C.c_int32),
                 ("transparency_layer_count", C.c_int32)]
+
+
+class AnimStep(C.Structure):
+    _fields_ = [("time", C.c_float), ("value", C.c_float * 4)]
+
+
+class AnimChannel(C.Structure):
+    _fields_ = [("node", C.c_int32), ("path", C.c_int32), ("n_steps", C.c_uint32),
+                ("steps", C.POINTER(AnimStep))]
+
+
+class Animation(C.Structure):
+    _fields_ = [("end_time", C.c_float), ("n_channels", C.c_uint32),
+                ("channels", C.POINTER(AnimChannel))]
 
 
 class FrameStats(C.Structure):

@@ -36,6 +36,13 @@ static int fail(int rc, const char *fmt, ...)
     return rc;
 }
 
+// shared with the asset loader (asset.cpp)
+int sw_fail_msg(int rc, const std::string &msg)
+{
+    g_err = msg;
+    return rc;
+}
+
 #define HIPCHK(x)                                                                   \
     do {                                                                            \
         hipError_t e_ = (x);                                                        \
@@ -638,8 +645,10 @@ extern "C" int swegl_scene_upload(swegl_ctx *c, const swegl_scene_desc *d, swegl
             return fail(SWEGL_EINVAL, "node %u primitive range out of range", ni);
         for (uint32_t k = 0; k < nd.n_primitives; k++) {
             const swegl_primitive_desc &p = d->primitives[nd.first_primitive + k];
-            if (p.mode != SWEGL_TRIANGLES && p.mode != SWEGL_TRIANGLE_STRIP && p.mode != SWEGL_TRIANGLE_FAN)
-                return fail(SWEGL_ENOTSUP, "primitive mode %d is not a triangle mode", p.mode);
+            // points and lines are kept but never drawn: the reference's marking
+            // and drawing loops handle the triangle modes only (renderer.cpp:83-185, 197-229)
+            if (p.mode < SWEGL_POINTS || p.mode > SWEGL_TRIANGLE_FAN)
+                return fail(SWEGL_EINVAL, "primitive mode %d is not a primitive_t::index_mode_t", p.mode);
             // material_id -1: pixel_shader_t takes default_material (pixel_shaders.cpp:14-31);
             // the texture shaders would index materials[-1], so renders with them refuse
             // such a scene (swegl_scene::no_material)

@@ -78,6 +78,18 @@ def lib():
             "swegl_camera_init": (None, [C.c_float, A.FP, A.FP, A.FP]),
             "swegl_camera_translate": (None, [A.FP, A.FP, C.c_float, C.c_float, C.c_float]),
             "swegl_camera_rotate": (None, [A.FP, C.c_int, C.c_float]),
+            "swegl_matrix_from_quaternion": (None, [A.FP, C.c_float, C.c_float, C.c_float,
+                                                    C.c_float]),
+            "swegl_asset_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+            "swegl_asset_scene": (C.c_int, [vp, C.POINTER(A.SceneDesc)]),
+            "swegl_asset_animations": (C.c_int, [vp, C.POINTER(C.POINTER(A.Animation)),
+                                                 C.POINTER(C.c_uint32)]),
+            "swegl_asset_destroy": (None, [vp]),
+            "swegl_animate": (C.c_int, [C.POINTER(A.Animation), C.c_uint32, C.c_float,
+                                        C.POINTER(A.NodeDesc), C.c_uint32]),
+            "swegl_image_read": (C.c_int, [C.c_char_p, C.POINTER(A.U32P), C.POINTER(C.c_uint32),
+                                           C.POINTER(C.c_uint32)]),
+            "swegl_image_free": (None, [A.U32P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -43,7 +43,9 @@ class primitive_t:
         n = len(self.indices)
         if self.mode == A.TRIANGLES:
             return n // 3
-        return max(0, n - 2)
+        if self.mode in (A.TRIANGLE_STRIP, A.TRIANGLE_FAN):
+            return max(0, n - 2)
+        return 0   # points and lines are not drawn
 
 
 class node_t:
@@ -84,6 +86,34 @@ class point_source_light:
         self.intensity = float(np.float32(intensity))
 
 
+class animation_channel_t:
+    """animation_channel_t (model.hpp:95-122): node, path (A.PATH_*) and the
+    keyframes sorted by time: times (n,) and values (n, 4) float32."""
+
+    def __init__(self, node_idx, path, times, values):
+        self.node_idx, self.path = int(node_idx), int(path)
+        self.times = np.ascontiguousarray(times, np.float32).reshape(-1)
+        self.values = np.ascontiguousarray(values, np.float32).reshape(-1, 4)
+        assert len(self.times) == len(self.values) and len(self.times) > 0
+
+
+class animation_t:
+    """animation_t (model.hpp:123-127)."""
+
+    def __init__(self, channels=None, end_time=None):
+        self.channels = list(channels or [])
+        self.end_time = float(np.float32(end_time if end_time is not None else
+                                         max([c.times[-1] for c in self.channels], default=0.0)))
+
+
+def from_quaternion(q0, q1, q2, q3):
+    """matrix44_t::from_quaternion (swegl/projection/matrix44.hpp:34-43); q0
+    is the scalar part."""
+    m = np.zeros((4, 4), np.float32)
+    lib().swegl_matrix_from_quaternion(A.fptr(m), q0, q1, q2, q3)
+    return m
+
+
 class scene_t:
     def __init__(self):
         self.nodes = []
@@ -95,7 +125,38 @@ class scene_t:
         self.point_source_lights = []
         self.materials = []
         self.images = []
+        self.animations = []
         self.generation = 0   # bump after editing meshes/textures in place
+
+    def animate(self, elapsed_seconds):
+        """scene_t::animate (model.hpp:146-177): pose the animated nodes for
+        elapsed_seconds (each animation loops over its end_time)."""
+        if not self.animations:
+            return
+        nodes = (A.NodeDesc * max(1, len(self.nodes)))()
+        for i, nd in enumerate(self.nodes):
+            nodes[i].scale[:] = [float(x) for x in np.asarray(nd.scale, np.float32)]
+            nodes[i].rotation[:] = [float(x) for x in np.asarray(nd.rotation, np.float32).reshape(16)]
+            nodes[i].translation[:] = [float(x) for x in np.asarray(nd.translation, np.float32)]
+        keep = []
+        anims = (A.Animation * len(self.animations))()
+        for a, an in enumerate(self.animations):
+            chans = (A.AnimChannel * max(1, len(an.channels)))()
+            for k, ch in enumerate(an.channels):
+                steps = (A.AnimStep * len(ch.times))()
+                for i in range(len(ch.times)):
+                    steps[i].time = float(ch.times[i])
+                    steps[i].value[:] = [float(v) for v in ch.values[i]]
+                chans[k] = A.AnimChannel(ch.node_idx, ch.path, len(ch.times), steps)
+                keep.append(steps)
+            keep.append(chans)
+            anims[a] = A.Animation(an.end_time, len(an.channels), chans)
+        check(lib().swegl_animate(anims, len(self.animations), float(elapsed_seconds), nodes,
+                                  len(self.nodes)), "swegl_animate")
+        for i, nd in enumerate(self.nodes):
+            nd.scale = np.array(nodes[i].scale[:], np.float32)
+            nd.rotation = np.array(nodes[i].rotation[:], np.float32).reshape(4, 4)
+            nd.translation = np.array(nodes[i].translation[:], np.float32)
 
     def n_triangles(self):
         return sum(p.n_triangles() for n in self.nodes for p in n.primitives)
