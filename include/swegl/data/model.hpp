@@ -205,4 +205,91 @@ inline node_t make_cube(float size, int material_idx)
     return node;
 }
 
+// make_tri (model.hpp:233-252): one triangle in the z = 0 plane.
+inline node_t make_tri(float size, int material_idx)
+{
+    node_t node;
+    primitive_t p;
+    const float pos[3][3] = {{0.0f, 0.0f, 0.0f}, {size, 0.0f, 0.0f}, {0.0f, size, 0.0f}};
+    const float tex[3][2] = {{0.0f, 0.0f}, {0.0f, 1.0f}, {1.0f, 0.0f}};
+    for (int i = 0; i < 3; i++) {
+        mesh_vertex_t mv;
+        mv.v = vertex_t(pos[i][0], pos[i][1], pos[i][2]);
+        mv.tex_coords = vec2f_t(tex[i][0], tex[i][1]);
+        mv.normal = normal_t(0, 0, 1);
+        p.vertices.push_back(mv);
+    }
+    p.indices = {0, 1, 2};
+    p.mode = primitive_t::TRIANGLES;
+    p.material_id = material_idx;
+    node.primitives.push_back(p);
+    return node;
+}
+
+namespace detail {
+// precision + 1 strips from swegl_make_tore / swegl_make_sphere
+template <typename F>
+inline node_t strips(unsigned int precision, int material_idx, F fill)
+{
+    const size_t n = precision + 1, V = n * 2 * n;
+    std::vector<float> pos(V * 3), nor(V * 3), tex(V * 2);
+    std::vector<uint32_t> idx(V);
+    if (fill(pos.data(), nor.data(), tex.data(), idx.data()) != SWEGL_OK)
+        throw std::runtime_error(std::string("swegl mesh generator: ") + swegl_last_error());
+    node_t node;
+    for (size_t k = 0; k < n; k++) {
+        primitive_t p;
+        for (size_t i = 2 * n * k; i < 2 * n * (k + 1); i++) {
+            mesh_vertex_t mv;
+            mv.v = vertex_t(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+            mv.normal.x() = nor[3 * i];   // already normal_t values
+            mv.normal.y() = nor[3 * i + 1];
+            mv.normal.z() = nor[3 * i + 2];
+            mv.tex_coords = vec2f_t(tex[2 * i], tex[2 * i + 1]);
+            p.vertices.push_back(mv);
+        }
+        p.indices.assign(idx.begin() + 2 * n * k, idx.begin() + 2 * n * (k + 1));
+        p.mode = primitive_t::TRIANGLE_STRIP;
+        p.material_id = material_idx;
+        node.primitives.push_back(std::move(p));
+    }
+    return node;
+}
+}  // namespace detail
+
+// make_tore (model.hpp:327-391)
+inline node_t make_tore(unsigned int precision, int material_idx)
+{
+    return detail::strips(precision, material_idx, [&](float *p, float *n, float *t, uint32_t *i) {
+        return swegl_make_tore(precision, p, n, t, i);
+    });
+}
+
+// make_sphere (model.hpp:393-464)
+inline node_t make_sphere(unsigned int precision, float radius, int material_idx)
+{
+    return detail::strips(precision, material_idx, [&](float *p, float *n, float *t, uint32_t *i) {
+        return swegl_make_sphere(precision, radius, p, n, t, i);
+    });
+}
+
+// calculate_face_normals (model.hpp:181-231)
+inline void calculate_face_normals(primitive_t &primitive)
+{
+    std::vector<float> pos, nor;
+    for (const mesh_vertex_t &v : primitive.vertices) {
+        pos.insert(pos.end(), {v.v.x(), v.v.y(), v.v.z()});
+        nor.insert(nor.end(), {v.normal.x(), v.normal.y(), v.normal.z()});
+    }
+    if (swegl_calculate_face_normals((int32_t)primitive.mode, pos.data(), nor.data(),
+                                     (uint32_t)primitive.vertices.size(), primitive.indices.data(),
+                                     (uint32_t)primitive.indices.size()) != SWEGL_OK)
+        throw std::runtime_error(std::string("swegl::calculate_face_normals: ") + swegl_last_error());
+    for (size_t i = 0; i < primitive.vertices.size(); i++) {
+        primitive.vertices[i].normal.x() = nor[3 * i];
+        primitive.vertices[i].normal.y() = nor[3 * i + 1];
+        primitive.vertices[i].normal.z() = nor[3 * i + 2];
+    }
+}
+
 }  // namespace swegl

@@ -82,8 +82,10 @@ struct scene_arrays {
         return r;
     }
 
-    explicit scene_arrays(const scene_t &s)
+    // with_meshes false: nodes, lights, ambient and sun only (a re-pose)
+    explicit scene_arrays(const scene_t &s, bool with_meshes = true)
     {
+        uint32_t n_prims = 0;
         for (const node_t &n : s.nodes) {
             swegl_node_desc nd{};
             nd.scale[0] = n.scale.x();
@@ -93,12 +95,15 @@ struct scene_arrays {
             nd.translation[0] = n.translation.x();
             nd.translation[1] = n.translation.y();
             nd.translation[2] = n.translation.z();
-            nd.first_primitive = (uint32_t)prims.size();
+            nd.first_primitive = n_prims;
             nd.n_primitives = (uint32_t)n.primitives.size();
+            n_prims += nd.n_primitives;
             nd.first_child = (uint32_t)children.size();
             nd.n_children = (uint32_t)n.children_idx.size();
             children.insert(children.end(), n.children_idx.begin(), n.children_idx.end());
             for (const primitive_t &p : n.primitives) {
+                if (!with_meshes)
+                    break;
                 std::vector<float> P, N, T;
                 for (const mesh_vertex_t &v : p.vertices) {
                     P.insert(P.end(), {v.v.x(), v.v.y(), v.v.z()});
@@ -125,6 +130,8 @@ struct scene_arrays {
         for (const material_t &m : s.materials)
             mats.push_back(mat(m));
         for (const texture_t &t : s.images) {
+            if (!with_meshes)
+                break;
             const mipmap_t &m = *t.m_mipmaps.at(0);
             texs.push_back({m.m_bitmap.data(), m.m_width, m.m_height});
         }
@@ -171,15 +178,17 @@ inline swegl_target *surface_target(device_state &st, SDL_Surface *s)
 inline void render_impl(scene_t &scene, viewport_t *const *vps, size_t n)
 {
     device_state &st = state();
-    scene_arrays arrays(scene);
     auto &se = st.scenes[&scene];
     if (!se.ds || se.gen != scene.device_generation) {
         if (se.ds)
             swegl_scene_destroy(st.ctx, se.ds);
         se.ds = nullptr;
+        scene_arrays arrays(scene);
         check(swegl_scene_upload(st.ctx, &arrays.d, &se.ds), "swegl_scene_upload");
         se.gen = scene.device_generation;
     } else {
+        // meshes and textures are resident: send the poses and lights only
+        scene_arrays arrays(scene, false);
         check(swegl_scene_update_pose(st.ctx, se.ds, &arrays.d), "swegl_scene_update_pose");
     }
     std::vector<swegl_viewport_desc> descs(n);
@@ -214,11 +223,13 @@ inline void render_impl(scene_t &scene, viewport_t *const *vps, size_t n)
         d.transparency_layer_count = vp.m_transparency_layer_count;
     }
     check(swegl_render(st.ctx, se.ds, descs.data(), (uint32_t)n, nullptr), "swegl_render");
-    // the viewport rectangles and z-buffers back into host memory
+    // the viewport rectangles and z-buffers back into host memory; with
+    // transparency layers the copy writes columns from 0 (post_shaders.hpp:24-28)
     for (size_t i = 0; i < n; i++) {
         viewport_t &vp = *vps[i];
-        char *dst = (char *)vp.m_screen->pixels + (size_t)vp.m_y * vp.m_screen->pitch + (size_t)vp.m_x * 4;
-        check(swegl_target_read_rect(st.ctx, descs[i].color, vp.m_x, vp.m_y, vp.m_w, vp.m_h, dst,
+        const int x0 = vp.m_transparency_layer_count > 0 ? 0 : vp.m_x;
+        char *dst = (char *)vp.m_screen->pixels + (size_t)vp.m_y * vp.m_screen->pitch + (size_t)x0 * 4;
+        check(swegl_target_read_rect(st.ctx, descs[i].color, x0, vp.m_y, vp.m_x + vp.m_w - x0, vp.m_h, dst,
                                      vp.m_screen->pitch),
               "swegl_target_read_rect");
         check(swegl_target_read(st.ctx, vp.m_device_depth, vp.m_zbuffer.get(), vp.m_w * 4), "swegl_target_read");

@@ -363,6 +363,23 @@ int  swegl_animate(const swegl_animation *animations, uint32_t n_animations, flo
 int  swegl_image_read(const char *filename, uint32_t **pixels, uint32_t *width, uint32_t *height);
 void swegl_image_free(uint32_t *pixels);
 
+/* ---- procedural meshes (swegl/data/model.hpp:181-466) ------------------ */
+/* make_tore(precision, material) (model.hpp:327-391) and
+ * make_sphere(precision, radius, material) (:393-464): precision + 1 triangle
+ * strips of 2 * (precision + 1) vertices each.  The caller's arrays hold
+ * V = (precision + 1) * 2 * (precision + 1) vertices: positions and normals
+ * V * 3 floats, tex_coords V * 2 floats, strip after strip; indices
+ * V uint32, 2 * (precision + 1) per strip, local to the strip. */
+int  swegl_make_tore(uint32_t precision, float *positions, float *normals, float *tex_coords,
+                     uint32_t *indices);
+int  swegl_make_sphere(uint32_t precision, float radius, float *positions, float *normals,
+                       float *tex_coords, uint32_t *indices);
+/* calculate_face_normals (model.hpp:181-231) on one primitive, in place,
+ * as the reference computes them (strips and fans; its last loop runs for
+ * fans, so triangle lists keep their normals). */
+int  swegl_calculate_face_normals(int32_t mode, const float *positions, float *normals,
+                                  uint32_t n_vertices, const uint32_t *indices, uint32_t n_indices);
+
 /* ---- host math helpers (same float semantics as the reference) --------- */
 /* matrix44_t::rotate_x/y/z (src/projection/matrix44.cpp:12-67), translate
  * (:87-92), swegl::scale (src/projection/points.cpp:14-30). */

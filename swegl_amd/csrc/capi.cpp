@@ -880,7 +880,8 @@ static void wave_tile(const swegl_ctx *c, const std::vector<JobIn> &jobs, int &t
     th = c->th;
     for (const JobIn &j : jobs)
         if (j.vp->transparency_layer_count > 0) {
-            tw = th = 16;   // the in-order layer raster works on 16x16 tiles
+            tw = SW_LAYER_TW;   // the in-order layer raster's tile
+            th = SW_LAYER_TH;
             return;
         }
     if (!c->tile_auto)
@@ -1058,15 +1059,13 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     } while (0)
     STAGE("vertex", launch_vertex(B, W.vblk, zero0, W.n_ctr - zero0, st));
     STAGE("mark", launch_mark(B, W.tblk, st));
-    STAGE("bin_count", launch_bin_count(B, W.tblk, st));   // also writes each triangle's tile rectangle
+    STAGE("bin_count", launch_bin_count(B, W.tblk, st));
+    STAGE("scan", launch_scan(B, st));
+    STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
     if (W.layers) {
-        // the layer raster walks each tile's triangles in draw order through
-        // their tile rectangles: no bins
         const uint32_t pixels = (uint32_t)(W.jd[0].V.vw * W.jd[0].V.vh);
         STAGE("raster", launch_layers(B, W.generic, pixels, st));
     } else {
-        STAGE("scan", launch_scan(B, st));
-        STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
         STAGE("raster", launch_raster(B, W.tw, W.th, groups, W.generic, st));
         STAGE("dof", launch_dof(B, W.dblk, W.rmax, st));
     }

@@ -45,6 +45,10 @@ struct TexDev {
 
 #define SW_MAX_DOF_RADIUS 16
 #define SW_MAX_LAYERS 4   // transparency layers per viewport (SWEGL_MAX_TRANSPARENCY_LAYERS)
+// tile of the in-order layer raster: 64 pixels, one wave
+#define SW_LAYER_TW 8
+#define SW_LAYER_TH 8
+#define SW_LAYER_THREADS 64
 // Depth-of-field output tile (pixels) of one 256-thread workgroup.
 #define SW_DOF_TW 64
 #define SW_DOF_TH 16

@@ -645,8 +645,11 @@ __device__ void vertex_attr(const JobDev &J, const NodeDev &N, const SVert &v, f
 // part (0 none, 1 flat, 2 phong) and a texture part (0 the material colour,
 // 1 nearest, 2 bilinear); otherwise it is SWEGL_SHADER_PHONG_BILINEAR, the
 // kernel variant the common case runs without the other branches.
+// Shading of triangle t's fragment with progress u on the row and span
+// already set up (S, R, sp).
 template <bool GENERIC>
-__device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uint32_t &oob)
+__device__ uint32_t shade_from(const JobDev &J, uint32_t t, const Setup &S, const Row &R, const Span &sp,
+                               float u, uint32_t &oob)
 {
     int lk = 2, tk = 2;
     bool combined = true;
@@ -656,20 +659,6 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
         tk = (k == 1 || k == 3 || k == 7) ? 2 : (k == 2 || k == 4 || k == 6) ? 1 : 0;
         combined = k >= 1 && k <= 4;
     }
-    const uint32_t t = seq >> 1, s = seq & 1u;
-    SubTri st[2];
-    Setup S;
-    Row R;
-    Span sp;
-    // The winner was produced by this very replay, so every step below
-    // succeeds; the guards only keep a corrupted key from looping.
-    if ((int)s >= tri_subs(J, t, st) || !setup_subtri(s ? st[1] : st[0], J.V, S, y) || !row_state(S, y, R) ||
-        !span_of(R, J.V, sp) || x < sp.x1 || x >= sp.x2)
-        return 0u;
-    Interp q = sp.q;
-    q.steps(x - sp.x1);
-    const float u = q.u();
-
     const PrimDev &P = J.prims[prim_of_tri(J, t)];
     const NodeDev &N = J.nodes[P.node];
     uint32_t one = P.color;
@@ -716,6 +705,26 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
         light = phong(center, normal, J.V.cam, J.ambient, J.sun, J.sun_I, J.lights, (int)J.n_lights);
     }
     return combined ? combine(color, light) : (uint32_t)light;
+}
+
+// Shading of the winning draw sequence seq at pixel (x, y): the sub-triangle,
+// row and span re-derived by the same replay that produced the winner.
+template <bool GENERIC>
+__device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uint32_t &oob)
+{
+    const uint32_t t = seq >> 1, s = seq & 1u;
+    SubTri st[2];
+    Setup S;
+    Row R;
+    Span sp;
+    // The winner was produced by this very replay, so every step below
+    // succeeds; the guards only keep a corrupted key from looping.
+    if ((int)s >= tri_subs(J, t, st) || !setup_subtri(s ? st[1] : st[0], J.V, S, y) || !row_state(S, y, R) ||
+        !span_of(R, J.V, sp) || x < sp.x1 || x >= sp.x2)
+        return 0u;
+    Interp q = sp.q;
+    q.steps(x - sp.x1);
+    return shade_from<GENERIC>(J, t, S, R, sp, q.u(), oob);
 }
 
 template <int TW, int TH, bool GENERIC>
@@ -939,18 +948,135 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // ------------------------------------------------------- k_raster_layers
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+
+// One pixel's base and layers (renderer.cpp:486-550)
+struct LayerPixel {
+    float bz, lz[SW_MAX_LAYERS];
+    uint32_t bc, bseq, lc[SW_MAX_LAYERS];
+};
+
+// The z test, shading and base or layer update of one fragment at a pixel
+// (renderer.cpp:489-550); `shade` computes the fragment's colour.
+template <typename ShadeF>
+__device__ __forceinline__ void layer_update(LayerPixel &P, float z, uint32_t seq, int K, uint32_t &frags,
+                                             uint32_t &nans, ShadeF shade)
+{
+    constexpr int KMAX = SW_MAX_LAYERS;
+    const float maxz = __uint_as_float(SW_MAX_Z_BITS);
+    if (z < SW_NEAR)
+        return;
+    frags++;
+    if (z != z) {
+        nans++;
+        return;
+    }
+    if (z >= P.bz)
+        return;
+    const uint32_t color = shade();
+    // the first layer that is empty or nearer than z (renderer.cpp:500-505)
+    int at = K;
+#pragma unroll
+    for (int k = KMAX - 1; k >= 0; k--)
+        if (k < K && (P.lz[k] == maxz || P.lz[k] < z))
+            at = k;
+    if ((color >> 24) == 255) {
+        // opaque: the new base; layers behind it drop out (:506-522)
+        P.bz = z;
+        P.bc = color;
+        P.bseq = seq;
+        float nz[KMAX];
+        uint32_t nc[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            const int from = k + at;
+            nz[k] = maxz;
+            nc[k] = 0u;
+#pragma unroll
+            for (int m = 0; m < KMAX; m++)
+                if (m == from && m < K) {
+                    nz[k] = P.lz[m];
+                    nc[k] = P.lc[m];
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            P.lz[k] = nz[k];
+            P.lc[k] = nc[k];
+        }
+        return;
+    }
+    // transparent: insert at `at` (:523-550)
+    bool all_used = false;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++)
+        if (k == K - 1)
+            all_used = P.lz[k] != maxz;
+    float cz = z;
+    uint32_t cc = color;
+    if (all_used) {
+        // shift toward layer 0: the new fragment lands at at-1, layer 0 falls out
+#pragma unroll
+        for (int k = KMAX - 1; k >= 0; k--)
+            if (k < at) {
+                const float tz = P.lz[k];
+                const uint32_t tc = P.lc[k];
+                P.lz[k] = cz;
+                P.lc[k] = cc;
+                cz = tz;
+                cc = tc;
+            }
+    } else {
+        // shift away from layer 0 up to the first empty layer
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < KMAX; k++)
+            if (!done && k >= at && k < K) {
+                const float tz = P.lz[k];
+                const uint32_t tc = P.lc[k];
+                P.lz[k] = cz;
+                P.lc[k] = cc;
+                cz = tz;
+                cc = tc;
+                done = cz == maxz;
+            }
+    }
+}
+
+// A chunk entry's sub-triangle staged in LDS: the vertices after
+// fill_triangle's z-sort when the triangle is drawn unclipped (the common
+// case); clipped triangles are re-derived by the lanes that need them.
+struct LayerSub {
+    float sx[3], sy[3], sz[3];
+    uint32_t a[3];
+    uint32_t kind;   // 0: not drawn, 1: one unclipped sub-triangle (+2: front), 4: re-derive
+};
+
 // Viewports with transparency layers (renderer.cpp:486-550).  The layer
-// updates depend on draw order, so each 16x16 tile walks the job's
-// triangles in draw order (those whose tile rectangle holds the tile, 256 at
-// a time, compacted in order; k_bin_count wrote the rectangles) and every lane owns one pixel: its base depth,
-// colour and sequence and its K layers live in registers, and the lane
-// applies each fragment of its pixel in order -- z test against the base,
-// shade, then the base write (opaque, alpha 255) with the farther layers
-// dropped, or the layer insertion (transparent).  Fragments of one triangle
-// fall on distinct pixels, so lanes never share state.  At the end the lane
-// writes the base colour, depth and sequence and the layers flattened into
-// layer 0 (viewport.cpp:80-84); k_tflatten/k_tcopy finish the flatten and
+// updates depend on the order of the fragments at a pixel, so each 8x8 tile
+// (one wave) applies its triangles in draw order: the tile's bin entries
+// and the job's big triangles whose rectangle holds the tile are gathered in
+// LDS and sorted by triangle index (bitonic sort; the index is the draw
+// order).  A tile with more than SW_LAYER_LIST entries walks all of the
+// job's triangles instead, a chunk at a time through their tile rectangles,
+// compacted in order.  Entries go in chunks of SW_LAYER_CHUNK (layer_chunk),
+// sized so that LDS lets about a dozen tiles share a CU: each lane
+// walks one entry's rows over the tile (the costly, independent part), then
+// each lane takes one pixel through the chunk's fragments in order -- z test
+// against the base, shade, then the base write (opaque, alpha 255) with the
+// farther layers dropped, or the layer insertion (transparent).  The base and
+// the K layers of each pixel live in registers.  At the end the lane writes
+// the base colour, depth and sequence and the layers flattened into layer 0
+// (viewport.cpp:80-84); k_tflatten/k_tcopy finish the flatten and
 // post_shader_t's copy (viewport.cpp:53-67, post_shaders.hpp:22-47).
+#define SW_LAYER_LIST 1024
+#define SW_LAYER_CHUNK 32
 __device__ __forceinline__ uint32_t blend_colors(uint32_t back, uint32_t front)
 {
     // colors.cpp:39-57: alpha from the front, integer alpha union, channels
@@ -971,55 +1097,122 @@ __device__ __forceinline__ uint32_t blend_colors(uint32_t back, uint32_t front)
     return out;
 }
 
-template <bool GENERIC>
-__global__ __launch_bounds__(256) void k_raster_layers(BatchDev B)
+// The fragments of an unclipped sub-triangle over the tile's rows
+// [ty0, ty0 + TH) and columns [tx0, tx0 + TW): the same row walk and span
+// stepping as raster_depth (renderer.cpp:462-558), writing each covered
+// pixel's depth to z[py * TW + px] and its bit to the returned mask.
+__device__ uint64_t layer_walk(const JobDev &J, const SubTri &st, int tx0, int ty0, float *z)
 {
-    constexpr int TW = 16, TH = 16, KMAX = SW_MAX_LAYERS;
-    __shared__ uint32_t s_list[256], s_wcount[4];
-    __shared__ uint32_t s_cnt[3];
-    const uint32_t g = blockIdx.x;
-    const int jidx = find_job<JTILE>(B, g);
-    const JobDev &J = B.jobs[jidx];
-    const int lt = (int)(g - J.tile_base);
-    const int tx = lt % J.tiles_x, ty = lt / J.tiles_x;
-    const int x = J.V.vx + tx * TW + (int)(threadIdx.x % TW), y = J.V.vy + ty * TH + (int)(threadIdx.x / TW);
-    const bool mine = x < J.V.vx + J.V.vw && y < J.V.vy + J.V.vh;
-    const int K = J.layers;
-    const float maxz = __uint_as_float(SW_MAX_Z_BITS);
-    float bz = maxz, lz[KMAX];
-    uint32_t bc = 0u, bseq = 0xFFFFFFFFu, lc[KMAX];
-#pragma unroll
-    for (int k = 0; k < KMAX; k++) {
-        lz[k] = maxz;
-        lc[k] = 0u;
+    constexpr int TW = SW_LAYER_TW, TH = SW_LAYER_TH;
+    Setup S;
+    if (!setup_subtri(st, J.V, S))
+        return 0ull;
+    const int wy0 = ty0, wy1 = min(ty0 + TH, J.V.vy + J.V.vh), wx1 = min(tx0 + TW, J.V.vx + J.V.vw);
+    uint64_t mask = 0ull;
+    int y = wy0;
+    while (y < wy1) {
+        int yend;
+        if (S.has_up && y >= S.up_y0 && y < S.up_y1) {
+            yend = min(S.up_y1, wy1);
+        } else if (S.has_lo && y >= S.lo_y0 && y < S.lo_y1) {
+            yend = min(S.lo_y1, wy1);
+        } else {
+            int nxt = wy1;
+            if (S.has_up && S.up_y0 > y && S.up_y1 > S.up_y0)
+                nxt = min(nxt, S.up_y0);
+            if (S.has_lo && S.lo_y0 > y && S.lo_y1 > S.lo_y0)
+                nxt = min(nxt, S.lo_y0);
+            y = nxt;
+            continue;
+        }
+        Row R;
+        row_state(S, y, R);
+        for (; y < yend; y++) {
+            Span sp;
+            if (span_of(R, J.V, sp)) {
+                const int xa = max(sp.x1, tx0), xb = min(sp.x2, wx1);
+                if (xa < xb) {
+                    Interp q = sp.q;
+                    q.steps(xa - sp.x1);
+                    for (int x = xa; x < xb; x++, q.step()) {
+                        const int p = (y - ty0) * TW + (x - tx0);
+                        z[p] = q.value(q.u());
+                        mask |= 1ull << p;
+                    }
+                }
+            }
+            R.left.steps(1);
+            R.right.steps(1);
+        }
     }
-    uint32_t frags = 0, nans = 0, oob = 0;
-    if (threadIdx.x < 3)
-        s_cnt[threadIdx.x] = 0;
-    const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    for (uint32_t base = 0; base < J.n_tris; base += 256) {
-        // the triangles of this batch whose tile rectangle holds the tile, in draw order
-        const uint32_t t = base + threadIdx.x;
-        bool cov = false;
-        if (t < J.n_tris) {
-            const uint2 r = J.rect[t];
-            cov = r.x != RECT_EMPTY && tx >= (int)(r.x & 0xFFFF) && tx <= (int)(r.y & 0xFFFF) &&
-                  ty >= (int)(r.x >> 16) && ty <= (int)(r.y >> 16);
+    return mask;
+}
+
+// Apply the entries list[0, n) (n <= SW_LAYER_CHUNK, in draw order) to the
+// tile.  Phase one, one lane per entry: the depth of every pixel the entry
+// covers goes to LDS (s_z, 64 depths per entry, and a coverage mask).
+// Phase two, one lane per pixel: the entries in order, each covered pixel's
+// fragment applied to the pixel's base and layers; a fragment that passes the
+// z test is shaded after setting its sub-triangle up again for its row.
+// Near-clipped triangles (two sub-triangles or clip vertices) are set up by
+// the pixel lanes themselves.
+template <bool GENERIC>
+__device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *list, uint32_t n, LayerSub *s_sub,
+                                            float *s_z, uint64_t *s_mask, int tx0, int ty0, int px, int py,
+                                            bool mine, int K, LayerPixel &P, uint32_t &frags, uint32_t &nans,
+                                            uint32_t &oob)
+{
+    constexpr int TW = SW_LAYER_TW;
+    if (threadIdx.x < n) {
+        const uint32_t t = list[threadIdx.x];
+        SubTri st[2];
+        const int ns = tri_subs(J, t, st);
+        LayerSub &L = s_sub[threadIdx.x];
+        uint64_t m = 0ull;
+        if (ns == 1 && !st[0].v[0].clip && !st[0].v[1].clip && !st[0].v[2].clip) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                L.sx[k] = st[0].v[k].s.x;
+                L.sy[k] = st[0].v[k].s.y;
+                L.sz[k] = st[0].v[k].s.z;
+                L.a[k] = st[0].v[k].a;
+            }
+            L.kind = st[0].front ? 3u : 1u;
+            m = layer_walk(J, st[0], tx0, ty0, s_z + threadIdx.x * 64);
+        } else {
+            L.kind = ns == 0 ? 0u : 4u;
         }
-        const uint64_t bal = __ballot(cov);
-        if (lane == 0)
-            s_wcount[wave] = (uint32_t)__popcll(bal);
-        __syncthreads();
-        uint32_t before = 0, total = 0;
-        for (int k = 0; k < 4; k++) {
-            before += k < wave ? s_wcount[k] : 0u;
-            total += s_wcount[k];
-        }
-        if (cov)
-            s_list[before + (uint32_t)__popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = t;
-        __syncthreads();
-        for (uint32_t i = 0; i < total && mine; i++) {
-            const uint32_t tt = s_list[i];
+        s_mask[threadIdx.x] = m;
+    }
+    __syncthreads();
+    const int x = tx0 + px, y = ty0 + py, p = py * TW + px;
+    for (uint32_t i = 0; i < n && mine; i++) {
+        const LayerSub &L = s_sub[i];
+        const uint32_t tt = list[i];
+        if (L.kind & 1u) {
+            if (!((s_mask[i] >> p) & 1ull))
+                continue;
+            layer_update(P, s_z[i * 64 + p], 2u * tt, K, frags, nans, [&] {
+                SubTri st;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    st.v[k].s = f3{L.sx[k], L.sy[k], L.sz[k]};
+                    st.v[k].a = st.v[k].b = L.a[k];
+                    st.v[k].cut = 0.0f;
+                    st.v[k].clip = 0;
+                }
+                st.front = (L.kind & 2u) != 0;
+                Setup S;
+                Row R;
+                Span sp;
+                // the walk above produced this fragment, so these succeed
+                if (!setup_subtri(st, J.V, S, y) || !row_state(S, y, R) || !span_of(R, J.V, sp))
+                    return 0u;
+                Interp q = sp.q;
+                q.steps(x - sp.x1);
+                return shade_from<GENERIC>(J, tt, S, R, sp, q.u(), oob);
+            });
+        } else if (L.kind == 4u) {
             SubTri st[2];
             const int ns = tri_subs(J, tt, st);
             for (int sub = 0; sub < ns; sub++) {
@@ -1031,118 +1224,150 @@ __global__ __launch_bounds__(256) void k_raster_layers(BatchDev B)
                     continue;
                 Interp q = sp.q;
                 q.steps(x - sp.x1);
-                float z = q.value(q.u());
-                if (z < SW_NEAR)
-                    continue;
-                frags++;
-                if (z != z) {
-                    nans++;
-                    continue;
-                }
-                if (z >= bz)
-                    continue;
-                const uint32_t seq = 2u * tt + (uint32_t)sub;
-                uint32_t color = shade_pixel<GENERIC>(J, seq, x, y, oob);
-                // the first layer that is empty or nearer than z (renderer.cpp:500-505)
-                int at = K;
-#pragma unroll
-                for (int k = KMAX - 1; k >= 0; k--)
-                    if (k < K && (lz[k] == maxz || lz[k] < z))
-                        at = k;
-                if ((color >> 24) == 255) {
-                    // opaque: the new base; layers behind it drop out (:506-522)
-                    bz = z;
-                    bc = color;
-                    bseq = seq;
-                    float nz[KMAX];
-                    uint32_t nc[KMAX];
-#pragma unroll
-                    for (int k = 0; k < KMAX; k++) {
-                        const int from = k + at;
-                        nz[k] = maxz;
-                        nc[k] = 0u;
-#pragma unroll
-                        for (int m = 0; m < KMAX; m++)
-                            if (m == from && m < K) {
-                                nz[k] = lz[m];
-                                nc[k] = lc[m];
-                            }
-                    }
-#pragma unroll
-                    for (int k = 0; k < KMAX; k++) {
-                        lz[k] = nz[k];
-                        lc[k] = nc[k];
-                    }
-                } else {
-                    // transparent: insert at `at` (:523-550)
-                    bool all_used = false;
-#pragma unroll
-                    for (int k = 0; k < KMAX; k++)
-                        if (k == K - 1)
-                            all_used = lz[k] != maxz;
-                    float cz = z;
-                    uint32_t cc = color;
-                    if (all_used) {
-                        // shift toward layer 0: the new fragment lands at at-1, layer 0 falls out
-#pragma unroll
-                        for (int k = KMAX - 1; k >= 0; k--)
-                            if (k < at) {
-                                const float tz = lz[k];
-                                const uint32_t tc = lc[k];
-                                lz[k] = cz;
-                                lc[k] = cc;
-                                cz = tz;
-                                cc = tc;
-                            }
-                    } else {
-                        // shift away from layer 0 up to the first empty layer
-                        bool done = false;
-#pragma unroll
-                        for (int k = 0; k < KMAX; k++)
-                            if (!done && k >= at && k < K) {
-                                const float tz = lz[k];
-                                const uint32_t tc = lc[k];
-                                lz[k] = cz;
-                                lc[k] = cc;
-                                cz = tz;
-                                cc = tc;
-                                done = cz == maxz;
-                            }
-                    }
-                }
+                const float u = q.u();
+                layer_update(P, q.value(u), 2u * tt + (uint32_t)sub, K, frags, nans,
+                             [&] { return shade_from<GENERIC>(J, tt, S, R, sp, u, oob); });
             }
         }
-        __syncthreads();   // s_list is rewritten by the next batch
+    }
+    __syncthreads();   // s_sub, s_z and s_mask are rewritten by the next chunk
+}
+
+template <bool GENERIC>
+__global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
+{
+    constexpr int TW = SW_LAYER_TW, TH = SW_LAYER_TH, KMAX = SW_MAX_LAYERS, NT = SW_LAYER_THREADS;
+    static_assert(TW * TH == 64 && NT == 64, "the layer raster works on 64-pixel tiles, one wave each");
+    constexpr uint32_t CH = SW_LAYER_CHUNK;
+    __shared__ uint32_t s_list[SW_LAYER_LIST], s_wcount[NT / 64];
+    __shared__ LayerSub s_sub[CH];
+    __shared__ float s_z[CH * 64];
+    __shared__ uint64_t s_mask[CH];
+    const uint32_t g = blockIdx.x;
+    const int jidx = find_job<JTILE>(B, g);
+    const JobDev &J = B.jobs[jidx];
+    const int lt = (int)(g - J.tile_base);
+    const int tx = lt % J.tiles_x, ty = lt / J.tiles_x;
+    const int tx0 = J.V.vx + tx * TW, ty0 = J.V.vy + ty * TH;
+    // this lane's pixel
+    const int px = (int)(threadIdx.x % TW), py = (int)(threadIdx.x / TW);
+    const bool mine = tx0 + px < J.V.vx + J.V.vw && ty0 + py < J.V.vy + J.V.vh;
+    const int K = J.layers;
+    const float maxz = __uint_as_float(SW_MAX_Z_BITS);
+    LayerPixel P;
+    P.bz = maxz;
+    P.bc = 0u;
+    P.bseq = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        P.lz[k] = maxz;
+        P.lc[k] = 0u;
+    }
+    uint32_t frags = 0, nans = 0, oob = 0;
+    const int lane = __lane_id();
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    auto covers = [&](uint32_t t) {
+        const uint2 r = J.rect[t];
+        return r.x != RECT_EMPTY && tx >= (int)(r.x & 0xFFFF) && tx <= (int)(r.y & 0xFFFF) &&
+               ty >= (int)(r.x >> 16) && ty <= (int)(r.y >> 16);
+    };
+    const uint32_t nb = B.tile_count[g * SW_TILE_PAD], off = B.tile_offset[g];
+    const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
+    if (nb + nbig <= SW_LAYER_LIST) {
+        // the tile's triangles: bin entries, then the big ones over this tile
+        for (uint32_t e = threadIdx.x; e < nb; e += NT)
+            s_list[e] = B.bins[off + e];
+        uint32_t total = nb;
+        const int wave = (int)(threadIdx.x >> 6);
+        for (uint32_t base = 0; base < nbig; base += NT) {
+            const uint32_t e = base + threadIdx.x;
+            const uint32_t t = e < nbig ? J.big[e] : 0u;
+            const bool cov = e < nbig && covers(t);
+            const uint64_t bal = __ballot(cov);
+            if (lane == 0)
+                s_wcount[wave] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t before = 0, sum = 0;
+            for (int k = 0; k < NT / 64; k++) {
+                before += k < wave ? s_wcount[k] : 0u;
+                sum += s_wcount[k];
+            }
+            if (cov)
+                s_list[total + before + (uint32_t)__popcll(bal & lt_mask)] = t;
+            total += sum;
+            __syncthreads();   // s_wcount is rewritten by the next pass
+        }
+        // bitonic sort of the first n2 >= total entries (padding sorts last)
+        uint32_t n2 = 1;
+        while (n2 < total)
+            n2 <<= 1;
+        for (uint32_t e = total + threadIdx.x; e < n2; e += NT)
+            s_list[e] = 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= n2; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < n2; i += NT) {
+                    const uint32_t o = i ^ j;
+                    if (o > i) {
+                        const uint32_t a = s_list[i], b = s_list[o];
+                        if ((a > b) == ((i & k) == 0)) {
+                            s_list[i] = b;
+                            s_list[o] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t c = 0; c < total; c += CH)
+            layer_chunk<GENERIC>(J, s_list + c, min(CH, total - c), s_sub, s_z, s_mask, tx0, ty0, px, py, mine, K, P,
+                                 frags, nans, oob);
+    } else {
+        // too many for LDS: every triangle of the job in draw order, CH at a time
+        const int wave = (int)(threadIdx.x >> 6);
+        for (uint32_t base = 0; base < J.n_tris; base += CH) {
+            const uint32_t t = base + threadIdx.x;
+            const bool cov = threadIdx.x < CH && t < J.n_tris && covers(t);
+            const uint64_t bal = __ballot(cov);
+            if (lane == 0)
+                s_wcount[wave] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+            for (int k = 0; k < NT / 64; k++) {
+                before += k < wave ? s_wcount[k] : 0u;
+                total += s_wcount[k];
+            }
+            if (cov)
+                s_list[before + (uint32_t)__popcll(bal & lt_mask)] = t;
+            __syncthreads();
+            layer_chunk<GENERIC>(J, s_list, total, s_sub, s_z, s_mask, tx0, ty0, px, py, mine, K, P, frags, nans,
+                                 oob);
+        }
     }
     if (mine) {
-        J.out_color[(int64_t)(y - J.out_oy) * J.out_pitch + (x - J.out_ox)] = bc;
+        const int x = tx0 + px, y = ty0 + py;
+        J.out_color[(int64_t)(y - J.out_oy) * J.out_pitch + (x - J.out_ox)] = P.bc;
         const int64_t zo = (int64_t)(y - J.V.vy) * J.V.vw + (x - J.V.vx);
-        J.zbuf[zo] = bz;
+        J.zbuf[zo] = P.bz;
         if (J.tid)
-            J.tid[zo] = bseq;
+            J.tid[zo] = P.bseq;
         // layers 1..K-1 flattened into layer 0 (viewport.cpp:43-51, 80-82)
-        uint32_t f = lc[0];
+        uint32_t f = P.lc[0];
 #pragma unroll
         for (int k = 1; k < KMAX; k++)
-            if (k < K && (lc[k] >> 24) != 0)
-                f = blend_colors(f, lc[k]);
+            if (k < K && (P.lc[k] >> 24) != 0)
+                f = blend_colors(f, P.lc[k]);
         J.tflat[zo] = f;
     }
     if (B.stats) {
-        if (frags)
-            atomicAdd(&s_cnt[0], frags);
-        if (nans)
-            atomicAdd(&s_cnt[1], nans);
-        if (oob)
-            atomicAdd(&s_cnt[2], oob);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            if (s_cnt[0])
-                stat_add(B, SW_ST_FRAGMENTS, s_cnt[0]);
-            if (s_cnt[1])
-                stat_add(B, SW_ST_NAN, s_cnt[1]);
-            if (s_cnt[2])
-                stat_add(B, SW_ST_OOB, s_cnt[2]);
+        // a wave sum, one atomic per counter and wave
+        const uint32_t f = wave_sum(frags), nn = wave_sum(nans), o = wave_sum(oob);
+        if (lane == 0) {
+            if (f)
+                stat_add(B, SW_ST_FRAGMENTS, f);
+            if (nn)
+                stat_add(B, SW_ST_NAN, nn);
+            if (o)
+                stat_add(B, SW_ST_OOB, o);
         }
     }
 }
@@ -1383,10 +1608,11 @@ hipError_t launch_layers(const BatchDev &B, bool generic, uint32_t pixels, hipSt
 {
     if (!B.n_tiles)
         return hipSuccess;
+    // one block per layer tile
     if (generic)
-        LAUNCH_CHECK(k_raster_layers<true>, dim3(B.n_tiles), dim3(256), 0, s, B);
+        LAUNCH_CHECK(k_raster_layers<true>, dim3(B.n_tiles), dim3(SW_LAYER_THREADS), 0, s, B);
     else
-        LAUNCH_CHECK(k_raster_layers<false>, dim3(B.n_tiles), dim3(256), 0, s, B);
+        LAUNCH_CHECK(k_raster_layers<false>, dim3(B.n_tiles), dim3(SW_LAYER_THREADS), 0, s, B);
     const uint32_t blocks = (pixels + 255) / 256;
     LAUNCH_CHECK(k_tflatten, dim3(blocks), dim3(256), 0, s, B, 0u);
     LAUNCH_CHECK(k_tcopy, dim3(blocks), dim3(256), 0, s, B, 0u);

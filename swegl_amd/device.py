@@ -90,6 +90,10 @@ def lib():
             "swegl_image_read": (C.c_int, [C.c_char_p, C.POINTER(A.U32P), C.POINTER(C.c_uint32),
                                            C.POINTER(C.c_uint32)]),
             "swegl_image_free": (None, [A.U32P]),
+            "swegl_make_tore": (C.c_int, [C.c_uint32, A.FP, A.FP, A.FP, A.U32P]),
+            "swegl_make_sphere": (C.c_int, [C.c_uint32, C.c_float, A.FP, A.FP, A.FP, A.U32P]),
+            "swegl_calculate_face_normals": (C.c_int, [C.c_int32, A.FP, A.FP, C.c_uint32, A.U32P,
+                                                       C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -68,6 +68,116 @@ def make_cube(size, material_idx):
     return node
 
 
+def make_tri(size, material_idx):
+    """make_tri (swegl/data/model.hpp:233-252): one triangle in the z = 0 plane."""
+    s = f32(size)
+    pos = np.array([[0, 0, 0], [s, 0, 0], [0, s, 0]], np.float32)
+    nor = np.tile(normal_t(0, 0, 1), (3, 1))
+    uv = np.array([[0, 0], [0, 1], [1, 0]], np.float32)
+    return node_t([primitive_t(pos, nor, uv, [0, 1, 2], A.TRIANGLES, material_idx)])
+
+
+def _strips(fill, precision, material_idx):
+    n = precision + 1
+    V = n * 2 * n
+    pos = np.zeros((V, 3), np.float32)
+    nor = np.zeros((V, 3), np.float32)
+    uv = np.zeros((V, 2), np.float32)
+    idx = np.zeros(V, np.uint32)
+    fill(pos, nor, uv, idx)
+    node = node_t()
+    for k in range(n):
+        sl = slice(2 * n * k, 2 * n * (k + 1))
+        node.primitives.append(primitive_t(pos[sl], nor[sl], uv[sl], idx[sl], A.TRIANGLE_STRIP,
+                                           material_idx))
+    return node
+
+
+def make_tore(precision, material_idx):
+    """make_tore (swegl/data/model.hpp:327-391): precision + 1 strips around a
+    torus of radii 2 and 0.8 about the y axis (library's swegl_make_tore)."""
+    from .device import check, lib
+
+    def fill(pos, nor, uv, idx):
+        check(lib().swegl_make_tore(int(precision), A.fptr(pos), A.fptr(nor), A.fptr(uv),
+                                    A.u32ptr(idx)), "make_tore")
+    return _strips(fill, int(precision), material_idx)
+
+
+def make_sphere(precision, radius, material_idx):
+    """make_sphere (swegl/data/model.hpp:393-464) (library's swegl_make_sphere)."""
+    from .device import check, lib
+
+    def fill(pos, nor, uv, idx):
+        check(lib().swegl_make_sphere(int(precision), float(f32(radius)), A.fptr(pos), A.fptr(nor),
+                                      A.fptr(uv), A.u32ptr(idx)), "make_sphere")
+    return _strips(fill, int(precision), material_idx)
+
+
+def calculate_face_normals(primitive):
+    """calculate_face_normals (swegl/data/model.hpp:181-231), in place."""
+    from .device import check, lib
+    check(lib().swegl_calculate_face_normals(primitive.mode, A.fptr(primitive.positions),
+                                             A.fptr(primitive.normals), len(primitive.positions),
+                                             A.u32ptr(primitive.indices), len(primitive.indices)),
+          "calculate_face_normals")
+
+
+def demo_scene(seed=0, tore_precision=100, sphere_precision=100):
+    """The reference demo's scene (src/test_1.cpp:50-148): a torus, a cube
+    stretched along x, a sphere, three overlapping triangles of which two are
+    translucent, two small cubes at the point lights; the three textures
+    (dice.bmp, tex.bmp, mercator.bmp there) are seeded random images of
+    256 x 256 here."""
+    rng = np.random.default_rng(seed)
+    s = scene_t()
+    s.images = [random_texture(rng, 256, 256) for _ in range(3)]
+    for im in s.images:
+        im.bitmap |= np.uint32(0xFF000000)
+    cols = [(128, 128, 128, 255, 0), (128, 128, 128, 255, 1), (128, 128, 128, 255, 2),
+            (128, 128, 255, 255, -1), (255, 128, 255, 255, -1), (128, 128, 255, 100, -1),
+            (128, 255, 128, 100, -1), (255, 128, 128, 100, -1)]
+    s.materials = [material_t(pixel_colors(b, g, r, a), 1, 1, t) for (b, g, r, a, t) in cols]
+    s.ambient_light_intensity = float(f32(0.2))
+    s.sun_direction = normal_t(*normal_t(1.0, -1.0, -1.0))
+    s.sun_intensity = float(f32(0.3))
+    s.point_source_lights = [point_source_light(np.array([0.0, 3.0, 0.0], np.float32), 0.6),
+                             point_source_light(np.array([0.5, 2.0, 0.0], np.float32), 100.0)]
+    tore = make_tore(tore_precision, 1)
+    tore.rotation = rotate_z(IDENTITY.copy(), 0.5)
+    tore.translation = np.array([0, 0, -2.5], np.float32)
+    cube = make_cube(1.0, 0)
+    cube.scale = np.array([2, 1, 1], np.float32)
+    sphere = make_sphere(sphere_precision, 2.0, 2)
+    sphere.translation = np.array([3, 0, -1], np.float32)
+    tri = make_tri(1, 5)
+    tri.translation = np.array([1, 0.5, 2.1], np.float32)
+    cube2 = make_cube(0.1, 4)
+    cube2.translation = s.point_source_lights[0].position.copy()
+    cube3 = make_cube(0.1, 4)
+    cube3.translation = s.point_source_lights[1].position.copy()
+    tri2 = make_tri(1, 6)
+    tri2.translation = np.array([1, 0.5, 2.0], np.float32)
+    tri3 = make_tri(1, 7)
+    tri3.translation = np.array([1, 0.5, 2.2], np.float32)
+    s.nodes = [tore, cube, sphere, tri, cube2, cube3, tri2, tri3]
+    s.root_nodes = list(range(len(s.nodes)))
+    return s
+
+
+def demo_viewport(width=800, height=600, layers=3):
+    """The demo's viewport (src/test_1.cpp:355-363): full window, Phong +
+    bilinear, `layers` transparency layers, null post shader, camera moved
+    back and turned."""
+    vp = viewport_t(0, 0, width, height, Surface(width, height), pixel_shader_light_and_texture(),
+                    layers)
+    vp.set_post_shader(post_shader_t())
+    vp.m_camera.translate(1, 2, -5)
+    vp.m_camera.rotate_y(-0.2)
+    vp.m_camera.rotate_x(-0.3)
+    return vp
+
+
 def random_texture(rng, w, h):
     """w x h pixel_colors with every byte i.i.d. uniform in [0, 255]."""
     return texture_t(rng.integers(0, 1 << 32, size=(h, w), dtype=np.uint64).astype(np.uint32))

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+from swegl_amd import _abi as A
 from swegl_amd import host, scenes
 from swegl_amd.device import Context
 from swegl_amd.host import pixel_colors
@@ -88,3 +89,48 @@ def test_layers_next_to_plain_viewports(ctx):
         vp.set_post_shader(host.post_shader_t())
         vps.append(vp)
     check(ctx, scene, vps)
+
+
+def crowded_scene(n, seed):
+    """n small triangles of random translucent materials over a few pixels
+    near the centre, one primitive each (draw order = index order)."""
+    rng = np.random.default_rng(seed)
+    s = host.scene_t()
+    s.materials = [host.material_t(pixel_colors(*rng.integers(0, 256, 3), int(a)))
+                   for a in rng.integers(20, 256, 16)]
+    c = rng.uniform(-0.1, 0.1, (n, 1, 3))
+    tri = (c + rng.uniform(-0.3, 0.3, (n, 3, 3))).astype(np.float32)
+    tri[..., 2] = rng.uniform(-0.5, 0.5, (n, 1))
+    mids = rng.integers(0, 16, n)
+    prims = [host.primitive_t(tri[k], np.tile([0, 0, 1], (3, 1)), np.zeros((3, 2)), [0, 1, 2],
+                              A.TRIANGLES, int(mids[k])) for k in range(n)]
+    s.nodes = [host.node_t(prims)]
+    s.root_nodes = [0]
+    return s
+
+
+@pytest.mark.parametrize("n", [300, 6000])
+def test_crowded_tiles(ctx, n):
+    """6000 triangles over the centre tiles exceed the sorted list in LDS
+    (2048 entries) and take the walk over all triangles; 300 take the sorted
+    bin list."""
+    s = crowded_scene(n, n)
+    surf = host.Surface(64, 48)
+    vp = host.viewport_t(0, 0, 64, 48, surf, host.pixel_shader_t(), 3)
+    vp.m_camera.translate(0, 0, -3)
+    vp.set_post_shader(host.post_shader_t())
+    got, want = check(ctx, s, [vp])
+    assert (want[0].pixels != 0).sum() > 40
+
+
+def test_big_triangles_merge_into_the_tile_lists(ctx):
+    """With a bin limit of 1 tile, every triangle over two tiles or more goes
+    to the job's big list, which the layer raster merges into each tile's
+    sorted list."""
+    scene, vps = scenes.cube_scene(9)
+    vps[0].transparency_layer_count = 2
+    ctx.set_bin_limit(1)
+    try:
+        check(ctx, scene, vps)
+    finally:
+        ctx.set_bin_limit(16)
