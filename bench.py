@@ -211,6 +211,44 @@ def bench_cube(ctx, A, check, L, seed, frames):
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}}, (scene, vps)
 
 
+def bench_demo(ctx, A, check, L, seed, frames):
+    """The reference demo's scene (src/test_1.cpp:50-148: torus and sphere of
+    20,200 triangles, a stretched cube, translucent triangles) at 800x600
+    with 3 transparency layers, as its frame loop draws it: frames per second
+    on the device, and with each frame read back to host memory (colour and
+    depth, the demo's SDL surface and z-buffer); the same without layers."""
+    from swegl_amd import scenes
+    from swegl_amd.device import DeviceScene, Target
+    from swegl_amd.host import SceneDescHolder
+
+    scene = scenes.demo_scene(seed)
+    ds = DeviceScene(ctx, SceneDescHolder(scene))
+    out = {"workload": f"demo scene {scene.n_triangles()} triangles, 800x600, seed {seed}"}
+    for layers in (3, 0):
+        vp = scenes.demo_viewport(layers=layers)
+        col = Target(ctx, A.TARGET_COLOR, 800, 600)
+        z = Target(ctx, A.TARGET_FLOAT, 800, 600)
+        d = (A.ViewportDesc * 1)()
+        d[0] = vp.desc(col, z)
+        for _ in range(5):
+            check(L.swegl_render(ctx.h, ds.h, d, 1, None), "swegl_render")
+        ctx.sync()
+        a = time.perf_counter()
+        for _ in range(frames):
+            check(L.swegl_render(ctx.h, ds.h, d, 1, None), "swegl_render")
+        ctx.sync()
+        dev = frames / (time.perf_counter() - a)
+        a = time.perf_counter()
+        for _ in range(frames // 4):
+            check(L.swegl_render(ctx.h, ds.h, d, 1, None), "swegl_render")
+            col.read()
+            z.read()
+        shown = (frames // 4) / (time.perf_counter() - a)
+        out[f"layers_{layers}"] = {"frames_per_s": dev, "frames_per_s_read_back": shown}
+    ds.close()
+    return out, scene
+
+
 def bench_batch(ctx, A, check, L, count, seed, iters):
     """Workload 'batch': `count` height fields (T log-uniform in [1e3, 2e5],
     square framebuffers of 256..1024, DoF on half)."""
@@ -335,6 +373,7 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="scenes in the batched workload (0 = skip)")
     ap.add_argument("--batch-iters", type=int, default=10)
     ap.add_argument("--heavy", type=int, default=64, help="scenes in the heavy workload (0 = skip)")
+    ap.add_argument("--demo", type=int, default=1, help="the demo scene leg (0 = skip)")
     ap.add_argument("--heavy-iters", type=int, default=3)
     ap.add_argument("--requests", type=int, default=20000, help="served requests per rate (0 = skip)")
     ap.add_argument("--lanes", type=int, default=3)
@@ -478,6 +517,8 @@ def main():
         "device": info,
     }
     out["cube"], cube = bench_cube(ctx, A, check, L, args.seed, 2000)
+    if args.demo:
+        out["demo"], demo_scene = bench_demo(ctx, A, check, L, args.seed, 400)
     if args.batch:
         if "batch" in tuning:
             ctx.apply_tuning(tuning["batch"])
@@ -492,6 +533,10 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, vps, args.cpu_seconds, args.cpu_threads)
         out["cpu_baseline_cube"] = cpu_baseline(cube[0], cube[1], 4.0, args.cpu_threads)
+        if args.demo:
+            from swegl_amd import scenes as _sc
+            out["cpu_baseline_demo"] = cpu_baseline(demo_scene, [_sc.demo_viewport(layers=3)], 4.0,
+                                                    args.cpu_threads)
         out["fp64_reshade"] = fp64_reshade(ctx, A, check, L, ds, scene, vp)
     print(json.dumps(out))
 

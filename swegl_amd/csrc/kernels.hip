@@ -707,6 +707,48 @@ __device__ uint32_t shade_from(const JobDev &J, uint32_t t, const Setup &S, cons
     return combined ? combine(color, light) : (uint32_t)light;
 }
 
+// The colour of triangle t's fragment before lighting: the material colour,
+// nearest texel or bilinear texel of the shader's texture part.  Its alpha
+// is the fragment's alpha for every shader whose light part only scales
+// b, g, r (a10, pixel_shaders.hpp:121-179) -- all but the light-only ones.
+template <bool GENERIC>
+__device__ uint32_t texture_from(const JobDev &J, uint32_t t, const Setup &S, const Row &R, const Span &sp,
+                                 float u, uint32_t &oob)
+{
+    int tk = 2;
+    if (GENERIC) {
+        const int k = J.shader;
+        tk = (k == 1 || k == 3 || k == 7) ? 2 : (k == 2 || k == 4 || k == 6) ? 1 : 0;
+    }
+    const PrimDev &P = J.prims[prim_of_tri(J, t)];
+    if (tk == 0)
+        return P.color;
+    const NodeDev &N = J.nodes[P.node];
+    uint32_t one = P.color;
+    const uint32_t *bm = &one;
+    int tw = 1, th = 1;
+    if (P.tex >= 0) {
+        TexDev T = J.texs[P.tex];
+        bm = J.texels + T.offset;
+        tw = T.w;
+        th = T.h;
+    }
+    TriAttr A;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        f3 w, n;
+        f2 tc;
+        vertex_attr(J, N, S.v[k], w, n, tc);
+        A.w[k] = w;
+        A.n[k] = n;
+        A.t[k] = f2{tc.x * (float)tw, tc.y * (float)th};
+    }
+    ScanShade SS = prepare_scanline(A, R.lower, R.lr, sp.pl, sp.pr);
+    if (tk == 1)
+        return nearest(bm, (uint32_t)tw, (uint32_t)th, SS.t_left + SS.t_dir * u);
+    return bilinear(bm, tw, th, SS.t_left + SS.t_dir * u, oob);
+}
+
 // Shading of the winning draw sequence seq at pixel (x, y): the sub-triangle,
 // row and span re-derived by the same replay that produced the winner.
 template <bool GENERIC>
@@ -965,8 +1007,8 @@ struct LayerPixel {
 // The z test, shading and base or layer update of one fragment at a pixel
 // (renderer.cpp:489-550); `shade` computes the fragment's colour.
 template <typename ShadeF>
-__device__ __forceinline__ void layer_update(LayerPixel &P, float z, uint32_t seq, int K, uint32_t &frags,
-                                             uint32_t &nans, ShadeF shade)
+__device__ __forceinline__ void layer_update(LayerPixel &P, float z, uint32_t seq, int K, bool deferred,
+                                             uint32_t &frags, uint32_t &nans, ShadeF shade)
 {
     constexpr int KMAX = SW_MAX_LAYERS;
     const float maxz = __uint_as_float(SW_MAX_Z_BITS);
@@ -980,6 +1022,8 @@ __device__ __forceinline__ void layer_update(LayerPixel &P, float z, uint32_t se
     if (z >= P.bz)
         return;
     const uint32_t color = shade();
+    // deferred: the slots keep the fragment's sequence, shaded at the end
+    const uint32_t keep = deferred ? seq : color;
     // the first layer that is empty or nearer than z (renderer.cpp:500-505)
     int at = K;
 #pragma unroll
@@ -989,7 +1033,7 @@ __device__ __forceinline__ void layer_update(LayerPixel &P, float z, uint32_t se
     if ((color >> 24) == 255) {
         // opaque: the new base; layers behind it drop out (:506-522)
         P.bz = z;
-        P.bc = color;
+        P.bc = keep;
         P.bseq = seq;
         float nz[KMAX];
         uint32_t nc[KMAX];
@@ -1019,7 +1063,7 @@ __device__ __forceinline__ void layer_update(LayerPixel &P, float z, uint32_t se
         if (k == K - 1)
             all_used = P.lz[k] != maxz;
     float cz = z;
-    uint32_t cc = color;
+    uint32_t cc = keep;
     if (all_used) {
         // shift toward layer 0: the new fragment lands at at-1, layer 0 falls out
 #pragma unroll
@@ -1065,8 +1109,8 @@ struct LayerSub {
 // LDS and sorted by triangle index (bitonic sort; the index is the draw
 // order).  A tile with more than SW_LAYER_LIST entries walks all of the
 // job's triangles instead, a chunk at a time through their tile rectangles,
-// compacted in order.  Entries go in chunks of SW_LAYER_CHUNK (layer_chunk),
-// sized so that LDS lets about a dozen tiles share a CU: each lane
+// compacted in order.  Entries go in chunks of SW_LAYER_CHUNK (layer_chunk):
+// each lane
 // walks one entry's rows over the tile (the costly, independent part), then
 // each lane takes one pixel through the chunk's fragments in order -- z test
 // against the base, shade, then the base write (opaque, alpha 255) with the
@@ -1076,7 +1120,7 @@ struct LayerSub {
 // (viewport.cpp:80-84); k_tflatten/k_tcopy finish the flatten and
 // post_shader_t's copy (viewport.cpp:53-67, post_shaders.hpp:22-47).
 #define SW_LAYER_LIST 1024
-#define SW_LAYER_CHUNK 32
+#define SW_LAYER_CHUNK 64
 __device__ __forceinline__ uint32_t blend_colors(uint32_t back, uint32_t front)
 {
     // colors.cpp:39-57: alpha from the front, integer alpha union, channels
@@ -1159,8 +1203,8 @@ __device__ uint64_t layer_walk(const JobDev &J, const SubTri &st, int tx0, int t
 template <bool GENERIC>
 __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *list, uint32_t n, LayerSub *s_sub,
                                             float *s_z, uint64_t *s_mask, int tx0, int ty0, int px, int py,
-                                            bool mine, int K, LayerPixel &P, uint32_t &frags, uint32_t &nans,
-                                            uint32_t &oob)
+                                            bool mine, int K, bool deferred, LayerPixel &P, uint32_t &frags,
+                                            uint32_t &nans, uint32_t &oob)
 {
     constexpr int TW = SW_LAYER_TW;
     if (threadIdx.x < n) {
@@ -1181,18 +1225,19 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
             m = layer_walk(J, st[0], tx0, ty0, s_z + threadIdx.x * 64);
         } else {
             L.kind = ns == 0 ? 0u : 4u;
+            m = ns == 0 ? 0ull : ~0ull;   // near-clipped: every pixel lane sets it up itself
         }
         s_mask[threadIdx.x] = m;
     }
     __syncthreads();
     const int x = tx0 + px, y = ty0 + py, p = py * TW + px;
     for (uint32_t i = 0; i < n && mine; i++) {
+        if (!((s_mask[i] >> p) & 1ull))
+            continue;
         const LayerSub &L = s_sub[i];
         const uint32_t tt = list[i];
         if (L.kind & 1u) {
-            if (!((s_mask[i] >> p) & 1ull))
-                continue;
-            layer_update(P, s_z[i * 64 + p], 2u * tt, K, frags, nans, [&] {
+            layer_update(P, s_z[i * 64 + p], 2u * tt, K, deferred, frags, nans, [&] {
                 SubTri st;
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
@@ -1210,7 +1255,9 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
                     return 0u;
                 Interp q = sp.q;
                 q.steps(x - sp.x1);
-                return shade_from<GENERIC>(J, tt, S, R, sp, q.u(), oob);
+                uint32_t no_count = 0;   // texel reads are counted where the survivors are shaded
+                return deferred ? texture_from<GENERIC>(J, tt, S, R, sp, q.u(), no_count)
+                                : shade_from<GENERIC>(J, tt, S, R, sp, q.u(), oob);
             });
         } else if (L.kind == 4u) {
             SubTri st[2];
@@ -1225,8 +1272,11 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
                 Interp q = sp.q;
                 q.steps(x - sp.x1);
                 const float u = q.u();
-                layer_update(P, q.value(u), 2u * tt + (uint32_t)sub, K, frags, nans,
-                             [&] { return shade_from<GENERIC>(J, tt, S, R, sp, u, oob); });
+                layer_update(P, q.value(u), 2u * tt + (uint32_t)sub, K, deferred, frags, nans, [&] {
+                    uint32_t no_count = 0;
+                    return deferred ? texture_from<GENERIC>(J, tt, S, R, sp, u, no_count)
+                                    : shade_from<GENERIC>(J, tt, S, R, sp, u, oob);
+                });
             }
         }
     }
@@ -1253,6 +1303,9 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
     const int px = (int)(threadIdx.x % TW), py = (int)(threadIdx.x / TW);
     const bool mine = tx0 + px < J.V.vx + J.V.vw && ty0 + py < J.V.vy + J.V.vh;
     const int K = J.layers;
+    // every shader but the light-only ones takes its alpha from the texture
+    // part: only the alpha is needed in order, the survivors are shaded last
+    const bool deferred = !GENERIC || (J.shader != 8 && J.shader != 9);   // 8, 9: lights_flat, lights_phong alone
     const float maxz = __uint_as_float(SW_MAX_Z_BITS);
     LayerPixel P;
     P.bz = maxz;
@@ -1319,8 +1372,8 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
                 __syncthreads();
             }
         for (uint32_t c = 0; c < total; c += CH)
-            layer_chunk<GENERIC>(J, s_list + c, min(CH, total - c), s_sub, s_z, s_mask, tx0, ty0, px, py, mine, K, P,
-                                 frags, nans, oob);
+            layer_chunk<GENERIC>(J, s_list + c, min(CH, total - c), s_sub, s_z, s_mask, tx0, ty0, px, py, mine, K,
+                                 deferred, P, frags, nans, oob);
     } else {
         // too many for LDS: every triangle of the job in draw order, CH at a time
         const int wave = (int)(threadIdx.x >> 6);
@@ -1339,9 +1392,18 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
             if (cov)
                 s_list[before + (uint32_t)__popcll(bal & lt_mask)] = t;
             __syncthreads();
-            layer_chunk<GENERIC>(J, s_list, total, s_sub, s_z, s_mask, tx0, ty0, px, py, mine, K, P, frags, nans,
-                                 oob);
+            layer_chunk<GENERIC>(J, s_list, total, s_sub, s_z, s_mask, tx0, ty0, px, py, mine, K, deferred, P,
+                                 frags, nans, oob);
         }
+    }
+    if (mine && deferred) {
+        // shade the base and the surviving layers (sequences in the slots)
+        const int x = tx0 + px, y = ty0 + py;
+        P.bc = P.bseq != 0xFFFFFFFFu ? shade_pixel<GENERIC>(J, P.bseq, x, y, oob) : 0u;
+#pragma unroll
+        for (int k = 0; k < KMAX; k++)
+            if (k < K)
+                P.lc[k] = P.lz[k] != maxz ? shade_pixel<GENERIC>(J, P.lc[k], x, y, oob) : 0u;
     }
     if (mine) {
         const int x = tx0 + px, y = ty0 + py;

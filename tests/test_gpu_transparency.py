@@ -70,8 +70,11 @@ def test_textured_cube_with_layers(ctx, seed, layers):
 def test_height_field_with_layers_and_other_shaders(ctx):
     scene, vps = scenes.heightfield_scene(3, n=40, width=160, height=120, tex=64, dof=False)
     vps[0].transparency_layer_count = 2
+    # the light-only shaders take their alpha from the light word, so they
+    # shade every fragment in order; the others shade the survivors at the end
     for sh in (host.pixel_shader_light_and_texture(), host.pixel_shader_texture(),
-               host.pixel_shader_light_and_texture(host.pixel_shader_lights_flat, host.pixel_shader_texture)):
+               host.pixel_shader_light_and_texture(host.pixel_shader_lights_flat, host.pixel_shader_texture),
+               host.pixel_shader_lights_phong(), host.pixel_shader_lights_flat(), host.pixel_shader_t()):
         vps[0].m_pixel_shader = sh
         check(ctx, scene, vps)
 
