@@ -374,6 +374,7 @@ def main():
     ap.add_argument("--batch-iters", type=int, default=10)
     ap.add_argument("--heavy", type=int, default=64, help="scenes in the heavy workload (0 = skip)")
     ap.add_argument("--demo", type=int, default=1, help="the demo scene leg (0 = skip)")
+    ap.add_argument("--cube", type=int, default=2000, help="frames of the cube leg (0 = skip)")
     ap.add_argument("--heavy-iters", type=int, default=3)
     ap.add_argument("--requests", type=int, default=20000, help="served requests per rate (0 = skip)")
     ap.add_argument("--lanes", type=int, default=3)
@@ -516,7 +517,9 @@ def main():
         "valu_roofline": valu,
         "device": info,
     }
-    out["cube"], cube = bench_cube(ctx, A, check, L, args.seed, 2000)
+    cube = None
+    if args.cube:
+        out["cube"], cube = bench_cube(ctx, A, check, L, args.seed, args.cube)
     if args.demo:
         out["demo"], demo_scene = bench_demo(ctx, A, check, L, args.seed, 400)
     if args.batch:
@@ -532,7 +535,8 @@ def main():
         out["heavy"] = bench_heavy(ctx, A, check, L, args.heavy, args.seed + 3, args.heavy_iters)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, vps, args.cpu_seconds, args.cpu_threads)
-        out["cpu_baseline_cube"] = cpu_baseline(cube[0], cube[1], 4.0, args.cpu_threads)
+        if cube is not None:
+            out["cpu_baseline_cube"] = cpu_baseline(cube[0], cube[1], 4.0, args.cpu_threads)
         if args.demo:
             from swegl_amd import scenes as _sc
             out["cpu_baseline_demo"] = cpu_baseline(demo_scene, [_sc.demo_viewport(layers=3)], 4.0,
