@@ -173,6 +173,8 @@ struct swegl_ctx {
     uint32_t groups_per_cu = 4;   // persistent raster workgroups per CU
     bool tile_auto = true;        // smaller tiles for waves with few of them (swegl_ctx_set_tile_auto)
     unsigned long long *h_stats = nullptr;   // pinned
+    void *h_stage = nullptr;                 // pinned staging for read-backs (grown on demand)
+    size_t h_stage_n = 0;
     bool prof = false;
     std::vector<ProfRec> pending;
     std::map<std::string, std::pair<double, uint64_t>> prof_acc;
@@ -335,6 +337,8 @@ extern "C" int swegl_ctx_destroy(swegl_ctx *c)
         (void)hipEventDestroy(e);
     if (c->h_stats)
         (void)hipHostFree(c->h_stats);
+    if (c->h_stage)
+        (void)hipHostFree(c->h_stage);
     c->graphs.clear();
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -483,15 +487,42 @@ extern "C" int swegl_target_destroy(swegl_ctx *c, swegl_target *t)
     return SWEGL_OK;
 }
 
+// Device rows to host memory through the context's pinned staging buffer:
+// one DMA into pinned memory, then a host copy, which is much faster than a
+// device copy straight into pageable memory (a frame read back every frame
+// is the presentation path of an interactive loop).
+static int read_rows(swegl_ctx *c, const char *src, size_t src_pitch, size_t row, int32_t h, void *dst,
+                     size_t pitch)
+{
+    const size_t n = row * (size_t)h;
+    if (n > c->h_stage_n) {
+        if (c->h_stage)
+            HIPCHK(hipHostFree(c->h_stage));
+        c->h_stage = nullptr;
+        c->h_stage_n = 0;
+        HIPCHK(hipHostMalloc(&c->h_stage, n, hipHostMallocDefault));
+        c->h_stage_n = n;
+    }
+    if (src_pitch == row)
+        HIPCHK(hipMemcpyAsync(c->h_stage, src, n, hipMemcpyDeviceToHost, c->stream));
+    else
+        HIPCHK(hipMemcpy2DAsync(c->h_stage, row, src, src_pitch, row, h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (pitch == row) {
+        memcpy(dst, c->h_stage, n);
+    } else {
+        for (int32_t r = 0; r < h; r++)
+            memcpy((char *)dst + (size_t)r * pitch, (const char *)c->h_stage + (size_t)r * row, row);
+    }
+    drain_profile(c);
+    return SWEGL_OK;
+}
+
 extern "C" int swegl_target_read(swegl_ctx *c, swegl_target *t, void *dst, int32_t pitch)
 {
     if (!c || !t || !dst || pitch < t->w * 4)
         return fail(SWEGL_EINVAL, "bad read arguments");
-    HIPCHK(hipMemcpy2DAsync(dst, pitch, t->buf.p, (size_t)t->w * 4, (size_t)t->w * 4, t->h,
-                            hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    drain_profile(c);
-    return SWEGL_OK;
+    return read_rows(c, (const char *)t->buf.p, (size_t)t->w * 4, (size_t)t->w * 4, t->h, dst, (size_t)pitch);
 }
 
 extern "C" int swegl_target_read_rect(swegl_ctx *c, swegl_target *t, int32_t x, int32_t y, int32_t w,
@@ -501,11 +532,7 @@ extern "C" int swegl_target_read_rect(swegl_ctx *c, swegl_target *t, int32_t x, 
         pitch < w * 4)
         return fail(SWEGL_EINVAL, "bad read_rect arguments");
     const char *src = (const char *)t->buf.p + ((size_t)y * t->w + x) * 4;
-    HIPCHK(hipMemcpy2DAsync(dst, pitch, src, (size_t)t->w * 4, (size_t)w * 4, h, hipMemcpyDeviceToHost,
-                            c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    drain_profile(c);
-    return SWEGL_OK;
+    return read_rows(c, src, (size_t)t->w * 4, (size_t)w * 4, h, dst, (size_t)pitch);
 }
 
 extern "C" int swegl_target_copy(swegl_ctx *c, swegl_target *dst, swegl_target *src)
