@@ -254,6 +254,15 @@ int  swegl_render(swegl_ctx *ctx, swegl_scene *scene,
                   const swegl_viewport_desc *viewports, uint32_t n_viewports,
                   swegl_frame_stats *stats);
 
+/* Parity aid: render one viewport (as swegl_render) and return the scene
+ * fields the reference's render writes into scene_t (model.hpp:20-29):
+ * v_world, normal_world and v_viewport (xyz per vertex) and yes, for every
+ * vertex in node, primitive, vertex order.  v_viewport is the frustum
+ * position, moved to screen coordinates for marked vertices as
+ * frustum_to_viewport does (vertex_shaders.hpp:72-84). */
+int  swegl_debug_vertex_state(swegl_ctx *ctx, swegl_scene *scene, const swegl_viewport_desc *viewport,
+                              float *v_world, float *normal_world, float *v_viewport, uint8_t *yes);
+
 /* Plain path as one replayed hipGraph: the first call for a given
  * (scene, viewport set) captures, later calls replay. */
 int  swegl_render_graph(swegl_ctx *ctx, swegl_scene *scene,

@@ -75,12 +75,7 @@ class Frame:
         self.color64 = color64
 
 
-def render(scene, viewports, surfaces=None, reshade64=False):
-    """swegl::render(scene, vp...) on the CPU oracle.  Each viewport renders
-    into a copy of its surface's current pixels (or `surfaces[i]`).  With
-    `reshade64` each frame also carries `color64` (h, w): the FP64 re-shade
-    of every pixel's final winner, before the post shader."""
-    holder = SceneDescHolder(scene)
+def _viewports(viewports, surfaces, reshade64):
     n = len(viewports)
     ovs = (OracleViewport * n)()
     keep = []
@@ -115,6 +110,17 @@ def render(scene, viewports, surfaces=None, reshade64=False):
         if ps is not None and ps.kind != A.POST_NULL:
             o.focal_distance, o.focal_depth = ps.focal_distance, ps.focal_depth
             o.dof_max_radius = ps.max_radius
+    return ovs, keep
+
+
+def render(scene, viewports, surfaces=None, reshade64=False):
+    """swegl::render(scene, vp...) on the CPU oracle.  Each viewport renders
+    into a copy of its surface's current pixels (or `surfaces[i]`).  With
+    `reshade64` each frame also carries `color64` (h, w): the FP64 re-shade
+    of every pixel's final winner, before the post shader."""
+    holder = SceneDescHolder(scene)
+    n = len(viewports)
+    ovs, keep = _viewports(viewports, surfaces, reshade64)
     st = (OracleStats * n)()
     rc = lib().oracle_render(C.byref(holder.desc), ovs, n, st)
     if rc != 0:
@@ -124,6 +130,26 @@ def render(scene, viewports, surfaces=None, reshade64=False):
         d = {k: getattr(st[i], k) for k, _ in OracleStats._fields_}
         frames.append(Frame(keep[4 * i], keep[4 * i + 1], keep[4 * i + 2], d, keep[4 * i + 3]))
     return frames
+
+
+def vertex_state(scene, viewport):
+    """The scene fields render() writes (model.hpp:20-29) after rendering
+    `viewport`: v_world, normal_world, v_viewport (n, 3) and yes (n,), the
+    vertices in node, primitive, vertex order."""
+    holder = SceneDescHolder(scene)
+    ovs, keep = _viewports([viewport], None, False)
+    n = scene.n_vertices()
+    out = {k: np.zeros((max(n, 1), 3), np.float32) for k in ("v_world", "normal_world", "v_viewport")}
+    yes = np.zeros(max(n, 1), np.uint8)
+    L = lib()
+    L.oracle_vertex_state.argtypes = [C.c_void_p, C.c_void_p, A.FP, A.FP, A.FP, C.POINTER(C.c_uint8)]
+    rc = L.oracle_vertex_state(C.byref(holder.desc), ovs, A.fptr(out["v_world"]), A.fptr(out["normal_world"]),
+                               A.fptr(out["v_viewport"]), yes.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc != 0:
+        raise RuntimeError(f"oracle_vertex_state failed: {A.ERRNAMES.get(rc, rc)}")
+    out = {k: v[:n] for k, v in out.items()}
+    out["yes"] = yes[:n]
+    return out
 
 
 def reshade_error(color, tid, color64):

@@ -49,6 +49,10 @@ typedef struct oracle_stats {
 /* swegl::render(scene, vp...) restated; stats[n] may be NULL. */
 int oracle_render(const swegl_scene_desc *scene, oracle_viewport *vps, uint32_t n,
                   oracle_stats *stats);
+/* v_world, normal_world, v_viewport (xyz per vertex) and yes after rendering
+ * one viewport, vertices in node, primitive, vertex order */
+int oracle_vertex_state(const swegl_scene_desc *scene, oracle_viewport *vp, float *v_world,
+                        float *normal_world, float *v_viewport, uint8_t *yes);
 
 /* ---- single pieces, exposed so tests can pin them in isolation ---------- */
 /* transform(v, m) (src/projection/points.cpp:8-13) */

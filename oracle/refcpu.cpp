@@ -1346,10 +1346,9 @@ void render_viewport(scene_m &sc, viewport_m &vp, const oracle_viewport &ov, ora
 /* ======================== exported entry points ========================= */
 extern "C" {
 
-int oracle_render(const swegl_scene_desc *scene, oracle_viewport *vps, uint32_t n,
-                  oracle_stats *stats)
+static int render_scene(const swegl_scene_desc *scene, oracle_viewport *vps, uint32_t n, oracle_stats *stats,
+                        scene_m &sc)
 {
-    scene_m sc;
     int rc = import_scene(scene, sc);
     if (rc)
         return rc;
@@ -1401,6 +1400,39 @@ int oracle_render(const swegl_scene_desc *scene, oracle_viewport *vps, uint32_t 
         vp.m11 = -o.h / 2.0f;
         render_viewport(sc, vp, o, stats ? &stats[i] : nullptr);
     }
+    return SWEGL_OK;
+}
+
+int oracle_render(const swegl_scene_desc *scene, oracle_viewport *vps, uint32_t n, oracle_stats *stats)
+{
+    scene_m sc;
+    return render_scene(scene, vps, n, stats, sc);
+}
+
+/* The scene fields render() writes (model.hpp:20-29) after rendering one
+ * viewport: v_world, normal_world, v_viewport and yes of every vertex, in
+ * node, primitive, vertex order (n_vertices entries; xyz triples). */
+int oracle_vertex_state(const swegl_scene_desc *scene, oracle_viewport *vp, float *v_world,
+                        float *normal_world, float *v_viewport, uint8_t *yes)
+{
+    scene_m sc;
+    int rc = render_scene(scene, vp, 1, nullptr, sc);
+    if (rc)
+        return rc;
+    size_t k = 0;
+    for (const node &nd : sc.nodes)
+        for (const primitive &pr : nd.prims)
+            for (const mesh_vertex &mv : pr.vertices) {
+                const vec3 *src[3] = {&mv.v_world, &mv.normal_world, &mv.v_viewport};
+                float *dst[3] = {v_world, normal_world, v_viewport};
+                for (int a = 0; a < 3; a++) {
+                    dst[a][3 * k] = src[a]->x;
+                    dst[a][3 * k + 1] = src[a]->y;
+                    dst[a][3 * k + 2] = src[a]->z;
+                }
+                yes[k] = mv.yes ? 1 : 0;
+                k++;
+            }
     return SWEGL_OK;
 }
 

@@ -1258,6 +1258,49 @@ extern "C" int swegl_render(swegl_ctx *c, swegl_scene *s, const swegl_viewport_d
     return render_jobs(c, c->lane0, jobs, stats);
 }
 
+// The scene fields the reference's render writes (model.hpp:20-29), for
+// parity tests: render one viewport, then read back the resident v_world and
+// normal_world and the viewport's frustum coordinates and marks, with the
+// viewport transform applied to the marked vertices as frustum_to_viewport
+// does (vertex_shaders.hpp:72-84).
+extern "C" int swegl_debug_vertex_state(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vp,
+                                        float *v_world, float *normal_world, float *v_viewport, uint8_t *yes)
+{
+    if (!c || !s || !vp || (s->n_vertices && (!v_world || !normal_world || !v_viewport || !yes)))
+        return fail(SWEGL_EINVAL, "null argument");
+    int rc = swegl_render(c, s, vp, 1, nullptr);
+    if (rc)
+        return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t V = s->n_vertices;
+    if (!V)
+        return SWEGL_OK;
+    if (s->views.empty())
+        return fail(SWEGL_EINVAL, "no vertex state for this scene");
+    const ViewScratch &vs = *s->views[0];
+    std::vector<float4> w(V), n(V), f(V);
+    std::vector<uint8_t> y(V);
+    HIPCHK(hipMemcpy(w.data(), s->world.p, V * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(n.data(), s->nworld.p, V * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(f.data(), vs.ndc.p, V * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(y.data(), vs.yes.p, V, hipMemcpyDeviceToHost));
+    float m00, m03, m11, m13;
+    hm::viewport_matrix(vp->x, vp->y, vp->w, vp->h, &m00, &m03, &m11, &m13);
+    for (size_t i = 0; i < V; i++) {
+        v_world[3 * i] = w[i].x;
+        v_world[3 * i + 1] = w[i].y;
+        v_world[3 * i + 2] = w[i].z;
+        normal_world[3 * i] = n[i].x;
+        normal_world[3 * i + 1] = n[i].y;
+        normal_world[3 * i + 2] = n[i].z;
+        yes[i] = y[i] ? 1 : 0;
+        v_viewport[3 * i] = y[i] ? m00 * f[i].x + m03 : f[i].x;
+        v_viewport[3 * i + 1] = y[i] ? m11 * f[i].y + m13 : f[i].y;
+        v_viewport[3 * i + 2] = f[i].z;
+    }
+    return SWEGL_OK;
+}
+
 extern "C" int swegl_render_graph(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vps,
                                   uint32_t n, swegl_frame_stats *stats)
 {

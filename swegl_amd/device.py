@@ -90,6 +90,8 @@ def lib():
             "swegl_image_read": (C.c_int, [C.c_char_p, C.POINTER(A.U32P), C.POINTER(C.c_uint32),
                                            C.POINTER(C.c_uint32)]),
             "swegl_image_free": (None, [A.U32P]),
+            "swegl_debug_vertex_state": (C.c_int, [vp, vp, C.POINTER(A.ViewportDesc), A.FP, A.FP, A.FP,
+                                                   C.POINTER(C.c_uint8)]),
             "swegl_make_tore": (C.c_int, [C.c_uint32, A.FP, A.FP, A.FP, A.U32P]),
             "swegl_make_sphere": (C.c_int, [C.c_uint32, C.c_float, A.FP, A.FP, A.FP, A.U32P]),
             "swegl_calculate_face_normals": (C.c_int, [C.c_int32, A.FP, A.FP, C.c_uint32, A.U32P,
