@@ -476,7 +476,7 @@ def main():
         except (OSError, ValueError):
             traffic = None
     valu = None
-    if valu_instr:
+    if valu_instr and f_vis:
         t = dom[1][0] * 1e-3   # live average launch duration, seconds
         valu = {"kernel": dom[0], "valu_instr_per_launch": valu_instr,
                 "achieved": valu_instr / t / 1e9, "peak": VALU_PEAK_G, "unit": "G wave64 VALU instr/s",
