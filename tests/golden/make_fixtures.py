@@ -20,6 +20,9 @@ CASES = [
 ] + [
     ("heightfield", dict(seed=11, n=64, width=160, height=120, tex=64, dof=False)),
     ("heightfield", dict(seed=12, n=128, width=256, height=192, tex=128, dof=True)),
+    ("demo", dict(seed=0, precision=20, width=200, height=150, layers=3)),
+    ("cube_layers", dict(seed=1, layers=2)),
+    ("cube_flat_nearest", dict(seed=2)),
 ]
 
 
@@ -34,8 +37,19 @@ def frames():
     for kind, kw in CASES:
         if kind == "cube":
             sc, vps = scenes.cube_scene(**kw)
-        else:
+        elif kind == "heightfield":
             sc, vps = scenes.heightfield_scene(**kw)
+        elif kind == "demo":
+            sc = scenes.demo_scene(kw["seed"], kw["precision"], kw["precision"])
+            vps = [scenes.demo_viewport(kw["width"], kw["height"], kw["layers"])]
+        elif kind == "cube_layers":
+            sc, vps = scenes.cube_scene(kw["seed"])
+            vps[0].transparency_layer_count = kw["layers"]
+        else:
+            from swegl_amd import host
+            sc, vps = scenes.cube_scene(kw["seed"])
+            vps[0].m_pixel_shader = host.pixel_shader_light_and_texture(host.pixel_shader_lights_flat,
+                                                                         host.pixel_shader_texture)
         fr = oracle.render(sc, vps)[0]
         out.append({"kind": kind, "args": kw, "pixels": digest(fr.pixels),
                     "zbuffer": digest(fr.zbuffer), "tid": digest(fr.tid),
