@@ -1036,15 +1036,26 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
     return SWEGL_OK;
 }
 
-// Size a resource set for a list of waves (grow-only).
-static int size_resources(Resources &R, std::vector<Wave> &waves)
+// Place each wave's job table in Resources::jobs; returns the bytes used.
+// Every call that uploads or copies job tables places them this way, so a
+// replayed graph finds each wave's table where it was captured.
+static size_t place_job_tables(std::vector<Wave> &waves)
 {
     size_t job_bytes = 0;
-    uint32_t tiles = 0, n_ctr = 0;
-    uint64_t cap = 0;
     for (Wave &W : waves) {
         W.job_offset = job_bytes;
         job_bytes += (W.jd.size() * sizeof(JobDev) + 255) & ~(size_t)255;
+    }
+    return job_bytes;
+}
+
+// Size a resource set for a list of waves (grow-only).
+static int size_resources(Resources &R, std::vector<Wave> &waves)
+{
+    const size_t job_bytes = place_job_tables(waves);
+    uint32_t tiles = 0, n_ctr = 0;
+    uint64_t cap = 0;
+    for (Wave &W : waves) {
         tiles = std::max(tiles, W.tiles);
         n_ctr = std::max(n_ctr, W.n_ctr);
         cap = std::max(cap, W.cap);
@@ -1229,6 +1240,11 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
         c->graphs[key] = std::move(g);
     } else {
         G = it->second.get();
+        // the same wave layout as the capture (the key says so): the same places
+        place_job_tables(waves);
+        for (const Wave &W : waves)
+            if (W.job_offset + W.jd.size() * sizeof(JobDev) > G->job_bytes)
+                return fail(SWEGL_EIO, "graph replay: job tables larger than at capture");
     }
     // write this frame's job tables into the next pinned buffer, once the
     // replay that last read it has finished

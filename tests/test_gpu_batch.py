@@ -86,3 +86,35 @@ def test_overlapping_viewports_keep_order(ctx):
     assert np.array_equal(got[1].pixels, want[1].pixels)
     for g, w in zip(got, want):
         compare(g, w)
+
+
+def test_multi_wave_graph_replays(ctx):
+    """Overlapping viewports on one surface make several launch waves in one
+    graph; every replay places each wave's job table where the capture put
+    it, so each replay equals the eager render."""
+    from swegl_amd.device import DeviceScene, Target
+    from swegl_amd.host import SceneDescHolder, pixel_shader_light_and_texture, post_shader_t, viewport_t
+    scene, vps = scenes.heightfield_scene(9, n=80, width=200, height=150, tex=64, dof=False)
+    surf = vps[0].m_screen
+    views = [vps[0]]
+    for k in range(2):
+        v = viewport_t(30 + 40 * k, 20 + 30 * k, 120, 100, surf, pixel_shader_light_and_texture(), 0)
+        v.m_camera.translate(0.2 * k, 1.0, -1.5)
+        v.m_camera.rotate_x(-0.5)
+        v.set_post_shader(post_shader_t())
+        views.append(v)
+    eager = gpu_render(ctx, scene, views)
+    ds = DeviceScene(ctx, SceneDescHolder(scene))
+    col = Target(ctx, A.TARGET_COLOR, surf.w, surf.h)
+    planes = [(Target(ctx, A.TARGET_FLOAT, v.m_w, v.m_h), Target(ctx, A.TARGET_UINT, v.m_w, v.m_h)) for v in views]
+    d = (A.ViewportDesc * len(views))()
+    for i, v in enumerate(views):
+        d[i] = v.desc(col, planes[i][0], planes[i][1])
+    for rep in range(4):
+        col.write(surf.pixels)
+        check(lib().swegl_render_graph(ctx.h, ds.h, d, len(views), None), "swegl_render_graph")
+        assert np.array_equal(col.read(), eager[0].pixels), rep
+        for i in range(len(views)):
+            assert np.array_equal(planes[i][1].read(), eager[i].tid), (rep, i)
+            assert np.array_equal(planes[i][0].read().view(np.uint32), eager[i].zbuffer.view(np.uint32)), (rep, i)
+    ds.close()

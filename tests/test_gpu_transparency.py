@@ -137,3 +137,26 @@ def test_big_triangles_merge_into_the_tile_lists(ctx):
         check(ctx, scene, vps)
     finally:
         ctx.set_bin_limit(16)
+
+
+def test_layered_jobs_in_batches_and_graph_replays(ctx):
+    """Layered viewports batched with plain ones (one graph, replayed three
+    times with eager renders between) equal each job rendered alone."""
+    from _helpers import Batch, same_frame
+    pairs = []
+    for k in range(4):
+        s, vps = scenes.cube_scene(20 + k, 160, 120)
+        vps[0].transparency_layer_count = k % 3
+        pairs.append((s, vps))
+    d = scenes.demo_scene(1, 16, 16)
+    pairs.append((d, [scenes.demo_viewport(200, 150, 3)]))
+    alone = [gpu_render(ctx, s, vps)[0] for s, vps in pairs]
+    b = Batch(ctx, pairs)
+    for rep in range(3):
+        b.render(graph=True)
+        got = b.frames()
+        for k in range(len(pairs)):
+            assert same_frame(got[k][0], alone[k]), (rep, k)
+        gpu_render(ctx, *pairs[0])     # eager work between the replays
+    want = oracle.render(*pairs[-1])[0]
+    compare(b.frames()[-1][0], want, colour_tol=1)
