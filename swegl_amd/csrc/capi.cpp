@@ -1192,6 +1192,9 @@ static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swe
 
 // ----------------------------------------------------------- graph path
 
+// A captured graph holds every launch argument by value: grids, tile shapes,
+// buffer capacities, LDS sizes and kernel variants.  The key names all of
+// them, so a replay runs only where the capture's launches fit.
 static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
 {
     std::string k = std::to_string(c->big_tiles) + "/" + std::to_string(c->groups_per_cu);
@@ -1200,6 +1203,9 @@ static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
              std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
              "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap) + "," +
              std::to_string(W.rmax) + (W.generic ? "g" : "") + (W.layers ? "L" : "");
+        // the layer waves' flatten and copy grids are sized by the viewport's pixels
+        if (W.layers)
+            k += std::to_string((uint64_t)W.jd[0].V.vw * W.jd[0].V.vh);
     }
     return k;
 }

@@ -160,3 +160,28 @@ def test_layered_jobs_in_batches_and_graph_replays(ctx):
         gpu_render(ctx, *pairs[0])     # eager work between the replays
     want = oracle.render(*pairs[-1])[0]
     compare(b.frames()[-1][0], want, colour_tol=1)
+
+
+def test_layered_graphs_keyed_by_pixel_count(ctx):
+    """Layered viewports of 17x17 and 24x24 have the same 3x3 grid of 8x8
+    layer tiles but need 2 and 3 blocks of the flatten and copy passes: each
+    takes a graph of its own, and every replay equals the eager render."""
+    from _helpers import Batch, same_frame
+    s, _ = scenes.cube_scene(5, 64, 48)
+    rng = np.random.default_rng(5)
+    pairs = []
+    for w in (17, 24):
+        surf = host.Surface(64, 48)
+        surf.pixels[:] = rng.integers(0, 1 << 32, (48, 64), dtype=np.uint64).astype(np.uint32)
+        vp = host.viewport_t(10, 8, w, w, surf, host.pixel_shader_light_and_texture(), 2)
+        vp.m_camera.translate(0, 0, -3)
+        vp.set_post_shader(host.post_shader_t())
+        pairs.append((s, [vp]))
+    alone = [gpu_render(ctx, sc, v)[0] for sc, v in pairs]
+    batches = [Batch(ctx, [p]) for p in pairs]
+    for rep in range(2):
+        for k, b in enumerate(batches):
+            b.targets[0][0][0].write(pairs[k][1][0].m_screen.pixels)
+            b.render(graph=True)
+            assert same_frame(b.frames()[0][0], alone[k]), (rep, k)
+    compare(alone[1], oracle.render(*pairs[1])[0], colour_tol=1)
