@@ -894,6 +894,7 @@ struct Wave {
     bool generic = false;     // some job uses a shader other than Phong + bilinear
     bool layers = false;      // the wave's (single) job has transparency layers
     uint64_t cap = 0;
+    uint32_t big_tiles = 0;   // bin limit of this wave (BatchDev::big_tiles)
     size_t job_offset = 0;   // byte offset of this wave's table in Resources::jobs
 };
 
@@ -1029,8 +1030,10 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         tris_total += s->n_tris;
     }
     W.n_ctr = SW_CTR_BIG0 + nj;
+    W.big_tiles = std::max<uint32_t>(1, W.layers ? std::max<uint32_t>(c->big_tiles, SW_LAYER_BIG_TILES)
+                                                 : c->big_tiles);
     // every binned triangle contributes at most big_tiles entries
-    W.cap = std::max<uint64_t>(4096, tris_total * std::max<uint32_t>(1, c->big_tiles));
+    W.cap = std::max<uint64_t>(4096, tris_total * W.big_tiles);
     if (W.cap > 0xFFFFFFF0ull)
         return fail(SWEGL_ERANGE, "too many triangles in one batch");
     return SWEGL_OK;
@@ -1082,7 +1085,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     B.bins = R.bins.as<uint32_t>();
     B.bin_capacity = (uint32_t)W.cap;
     B.ctr = R.ctr.as<uint32_t>();
-    B.big_tiles = c->big_tiles;
+    B.big_tiles = W.big_tiles;
     B.stats = want_stats ? 1u : 0u;
     // statistics are zeroed once per render by the first wave; control words
     // and big-list lengths by every wave
@@ -1202,7 +1205,8 @@ static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
         k += "|" + std::to_string(W.tw) + "x" + std::to_string(W.th) + ":" + std::to_string(W.jd.size()) + ":" +
              std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
              "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap) + "," +
-             std::to_string(W.rmax) + (W.generic ? "g" : "") + (W.layers ? "L" : "");
+             std::to_string(W.rmax) + "," + std::to_string(W.big_tiles) + (W.generic ? "g" : "") +
+             (W.layers ? "L" : "");
         // the layer waves' flatten and copy grids are sized by the viewport's pixels
         if (W.layers)
             k += std::to_string((uint64_t)W.jd[0].V.vw * W.jd[0].V.vh);

@@ -49,6 +49,9 @@ struct TexDev {
 #define SW_LAYER_TW 8
 #define SW_LAYER_TH 8
 #define SW_LAYER_THREADS 64
+// Bin limit of layered waves: on 8x8 tiles a smaller limit would send most
+// triangles of a close-up mesh to the big list, which every tile scans.
+#define SW_LAYER_BIG_TILES 32
 // Depth-of-field output tile (pixels) of one 256-thread workgroup.
 #define SW_DOF_TW 64
 #define SW_DOF_TH 16

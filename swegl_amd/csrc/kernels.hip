@@ -1107,7 +1107,7 @@ struct LayerSub {
 // (one wave) applies its triangles in draw order: the tile's bin entries
 // and the job's big triangles whose rectangle holds the tile are gathered in
 // LDS and sorted by triangle index (bitonic sort; the index is the draw
-// order).  A tile with more than SW_LAYER_LIST entries walks all of the
+// order).  A tile with more than SW_LAYER_LIST such entries walks all of the
 // job's triangles instead, a chunk at a time through their tile rectangles,
 // compacted in order.  Entries go in chunks of SW_LAYER_CHUNK (layer_chunk):
 // each lane
@@ -1326,13 +1326,16 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
     };
     const uint32_t nb = B.tile_count[g * SW_TILE_PAD], off = B.tile_offset[g];
     const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
-    if (nb + nbig <= SW_LAYER_LIST) {
-        // the tile's triangles: bin entries, then the big ones over this tile
+    // the tile's triangles: bin entries, then the job's big ones whose
+    // rectangle holds this tile, while they fit in LDS
+    bool listed = nb <= SW_LAYER_LIST;
+    uint32_t total = 0;
+    if (listed) {
         for (uint32_t e = threadIdx.x; e < nb; e += NT)
             s_list[e] = B.bins[off + e];
-        uint32_t total = nb;
+        total = nb;
         const int wave = (int)(threadIdx.x >> 6);
-        for (uint32_t base = 0; base < nbig; base += NT) {
+        for (uint32_t base = 0; base < nbig && listed; base += NT) {
             const uint32_t e = base + threadIdx.x;
             const uint32_t t = e < nbig ? J.big[e] : 0u;
             const bool cov = e < nbig && covers(t);
@@ -1345,11 +1348,15 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
                 before += k < wave ? s_wcount[k] : 0u;
                 sum += s_wcount[k];
             }
-            if (cov)
-                s_list[total + before + (uint32_t)__popcll(bal & lt_mask)] = t;
+            const uint32_t at = total + before + (uint32_t)__popcll(bal & lt_mask);
+            if (cov && at < SW_LAYER_LIST)
+                s_list[at] = t;
             total += sum;
+            listed = total <= SW_LAYER_LIST;   // the same for every lane
             __syncthreads();   // s_wcount is rewritten by the next pass
         }
+    }
+    if (listed) {
         // bitonic sort of the first n2 >= total entries (padding sorts last)
         uint32_t n2 = 1;
         while (n2 < total)
