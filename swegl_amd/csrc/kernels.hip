@@ -1141,16 +1141,49 @@ __device__ __forceinline__ uint32_t blend_colors(uint32_t back, uint32_t front)
     return out;
 }
 
-// The fragments of an unclipped sub-triangle over the tile's rows
+// The fragments of triangle t's unclipped sub-triangle over the tile's rows
 // [ty0, ty0 + TH) and columns [tx0, tx0 + TW): the same row walk and span
 // stepping as raster_depth (renderer.cpp:462-558), writing each covered
-// pixel's depth to z[py * TW + px] and its bit to the returned mask.
-__device__ uint64_t layer_walk(const JobDev &J, const SubTri &st, int tx0, int ty0, float *z)
+// pixel's depth to z[py * TW + px] and its bit to the returned mask.  With
+// `opaque`, also the bit of every covered pixel whose texture-part colour
+// (texture_from: material colour, nearest or bilinear texel) has alpha 255,
+// from the same row and span state a re-derivation for that pixel gives.
+template <bool GENERIC>
+__device__ uint64_t layer_walk(const JobDev &J, uint32_t t, const SubTri &st, int tx0, int ty0, float *z,
+                               uint64_t *opaque)
 {
     constexpr int TW = SW_LAYER_TW, TH = SW_LAYER_TH;
     Setup S;
     if (!setup_subtri(st, J.V, S))
         return 0ull;
+    // texture part of the colour (texture_from)
+    int tk = 2;
+    if (GENERIC) {
+        const int k = J.shader;
+        tk = (k == 1 || k == 3 || k == 7) ? 2 : (k == 2 || k == 4 || k == 6) ? 1 : 0;
+    }
+    const PrimDev &P = J.prims[prim_of_tri(J, t)];
+    uint32_t one = P.color;
+    const uint32_t *bm = &one;
+    int tw = 1, th = 1;
+    TriAttr A;
+    uint64_t opq = 0ull;
+    const bool textured = opaque != nullptr && tk != 0;
+    if (textured) {
+        if (P.tex >= 0) {
+            TexDev T = J.texs[P.tex];
+            bm = J.texels + T.offset;
+            tw = T.w;
+            th = T.h;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            A.w[k] = f3{0.0f, 0.0f, 0.0f};
+            A.n[k] = f3{0.0f, 0.0f, 0.0f};
+            A.t[k] = f2{J.tu[S.v[k].a] * (float)tw, J.tv[S.v[k].a] * (float)th};
+        }
+    }
+    const bool flat_opaque = (P.color >> 24) == 255;   // tk == 0: the material colour
     const int wy0 = ty0, wy1 = min(ty0 + TH, J.V.vy + J.V.vh), wx1 = min(tx0 + TW, J.V.vx + J.V.vw);
     uint64_t mask = 0ull;
     int y = wy0;
@@ -1178,10 +1211,22 @@ __device__ uint64_t layer_walk(const JobDev &J, const SubTri &st, int tx0, int t
                 if (xa < xb) {
                     Interp q = sp.q;
                     q.steps(xa - sp.x1);
+                    ScanShade SS;
+                    if (textured)
+                        SS = prepare_scanline(A, R.lower, R.lr, sp.pl, sp.pr);
                     for (int x = xa; x < xb; x++, q.step()) {
                         const int p = (y - ty0) * TW + (x - tx0);
-                        z[p] = q.value(q.u());
+                        const float u = q.u();
+                        z[p] = q.value(u);
                         mask |= 1ull << p;
+                        bool op = flat_opaque;
+                        if (textured) {
+                            const f2 tc = SS.t_left + SS.t_dir * u;
+                            uint32_t no_count = 0;   // texel reads are counted where the survivors are shaded
+                            op = tk == 1 ? (nearest(bm, (uint32_t)tw, (uint32_t)th, tc) >> 24) == 255
+                                         : bilinear_channel(bilinear_taps(bm, tw, th, tc, no_count), 3) == 255;
+                        }
+                        opq |= (uint64_t)op << p;
                     }
                 }
             }
@@ -1189,6 +1234,8 @@ __device__ uint64_t layer_walk(const JobDev &J, const SubTri &st, int tx0, int t
             R.right.steps(1);
         }
     }
+    if (opaque)
+        *opaque = opq;
     return mask;
 }
 
@@ -1202,9 +1249,9 @@ __device__ uint64_t layer_walk(const JobDev &J, const SubTri &st, int tx0, int t
 // the pixel lanes themselves.
 template <bool GENERIC>
 __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *list, uint32_t n, LayerSub *s_sub,
-                                            float *s_z, uint64_t *s_mask, int tx0, int ty0, int px, int py,
-                                            bool mine, int K, bool deferred, LayerPixel &P, uint32_t &frags,
-                                            uint32_t &nans, uint32_t &oob)
+                                            float *s_z, uint64_t *s_mask, uint64_t *s_opq, int tx0, int ty0,
+                                            int px, int py, bool mine, int K, bool deferred, LayerPixel &P,
+                                            uint32_t &frags, uint32_t &nans, uint32_t &oob)
 {
     constexpr int TW = SW_LAYER_TW;
     if (threadIdx.x < n) {
@@ -1222,7 +1269,8 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
                 L.a[k] = st[0].v[k].a;
             }
             L.kind = st[0].front ? 3u : 1u;
-            m = layer_walk(J, st[0], tx0, ty0, s_z + threadIdx.x * 64);
+            m = layer_walk<GENERIC>(J, t, st[0], tx0, ty0, s_z + threadIdx.x * 64,
+                                    deferred ? s_opq + threadIdx.x : nullptr);
         } else {
             L.kind = ns == 0 ? 0u : 4u;
             m = ns == 0 ? 0ull : ~0ull;   // near-clipped: every pixel lane sets it up itself
@@ -1238,6 +1286,9 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
         const uint32_t tt = list[i];
         if (L.kind & 1u) {
             layer_update(P, s_z[i * 64 + p], 2u * tt, K, deferred, frags, nans, [&] {
+                // deferred: only the alpha matters here, found by the walk
+                if (deferred)
+                    return ((s_opq[i] >> p) & 1ull) ? 0xFF000000u : 0u;
                 SubTri st;
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
@@ -1255,9 +1306,7 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
                     return 0u;
                 Interp q = sp.q;
                 q.steps(x - sp.x1);
-                uint32_t no_count = 0;   // texel reads are counted where the survivors are shaded
-                return deferred ? texture_from<GENERIC>(J, tt, S, R, sp, q.u(), no_count)
-                                : shade_from<GENERIC>(J, tt, S, R, sp, q.u(), oob);
+                return shade_from<GENERIC>(J, tt, S, R, sp, q.u(), oob);
             });
         } else if (L.kind == 4u) {
             SubTri st[2];
@@ -1292,7 +1341,7 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
     __shared__ uint32_t s_list[SW_LAYER_LIST], s_wcount[NT / 64];
     __shared__ LayerSub s_sub[CH];
     __shared__ float s_z[CH * 64];
-    __shared__ uint64_t s_mask[CH];
+    __shared__ uint64_t s_mask[CH], s_opq[CH];
     const uint32_t g = blockIdx.x;
     const int jidx = find_job<JTILE>(B, g);
     const JobDev &J = B.jobs[jidx];
@@ -1379,7 +1428,7 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
                 __syncthreads();
             }
         for (uint32_t c = 0; c < total; c += CH)
-            layer_chunk<GENERIC>(J, s_list + c, min(CH, total - c), s_sub, s_z, s_mask, tx0, ty0, px, py, mine, K,
+            layer_chunk<GENERIC>(J, s_list + c, min(CH, total - c), s_sub, s_z, s_mask, s_opq, tx0, ty0, px, py, mine, K,
                                  deferred, P, frags, nans, oob);
     } else {
         // too many for LDS: every triangle of the job in draw order, CH at a time
@@ -1399,7 +1448,7 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
             if (cov)
                 s_list[before + (uint32_t)__popcll(bal & lt_mask)] = t;
             __syncthreads();
-            layer_chunk<GENERIC>(J, s_list, total, s_sub, s_z, s_mask, tx0, ty0, px, py, mine, K, deferred, P,
+            layer_chunk<GENERIC>(J, s_list, total, s_sub, s_z, s_mask, s_opq, tx0, ty0, px, py, mine, K, deferred, P,
                                  frags, nans, oob);
         }
     }

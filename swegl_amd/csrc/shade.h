@@ -85,7 +85,13 @@ SW_D int rem(int x, int m)
     return x % m;
 }
 
-SW_D uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
+// The four texels and weights of one bilinear sample.
+struct BilTaps {
+    uint32_t p[4];
+    float w[4];
+};
+
+SW_D BilTaps bilinear_taps(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
 {
     float v = t.x;
     float u = t.y;
@@ -111,7 +117,7 @@ SW_D uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
     const int64_t lim = (int64_t)tw * th;
     int64_t idx[4] = {(int64_t)v1m + u1m, (int64_t)v2m + u1m, (int64_t)v1m + u2m,
                       (int64_t)v2m + u2m};
-    uint32_t p[4];
+    BilTaps T;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         int64_t i = idx[k];
@@ -119,22 +125,33 @@ SW_D uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
             oob++;
             i = i < 0 ? 0 : lim - 1;
         }
-        p[k] = pc[i];
+        T.p[k] = pc[i];
     }
-    const float w0 = (u - u1) * (v - v1);
-    const float w1 = (u - u1) * (v2 - v);
-    const float w2 = (u2 - u) * (v - v1);
-    const float w3 = (u2 - u) * (v2 - v);
+    T.w[0] = (u - u1) * (v - v1);
+    T.w[1] = (u - u1) * (v2 - v);
+    T.w[2] = (u2 - u) * (v - v1);
+    T.w[3] = (u2 - u) * (v2 - v);
+    return T;
+}
+
+// Channel c (0 b, 1 g, 2 r, 3 alpha) of a bilinear sample.
+SW_D uint32_t bilinear_channel(const BilTaps &T, int c)
+{
+    float a = (float)((T.p[0] >> (8 * c)) & 0xFF) * T.w[0];
+    float b = (float)((T.p[1] >> (8 * c)) & 0xFF) * T.w[1];
+    float d = (float)((T.p[2] >> (8 * c)) & 0xFF) * T.w[2];
+    float e = (float)((T.p[3] >> (8 * c)) & 0xFF) * T.w[3];
+    float s = ((a + b) + d) + e;
+    return (uint32_t)to_u8(roundf(s));
+}
+
+SW_D uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
+{
+    const BilTaps T = bilinear_taps(pc, tw, th, t, oob);
     uint32_t out = 0;
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        float a = (float)((p[0] >> (8 * c)) & 0xFF) * w0;
-        float b = (float)((p[1] >> (8 * c)) & 0xFF) * w1;
-        float d = (float)((p[2] >> (8 * c)) & 0xFF) * w2;
-        float e = (float)((p[3] >> (8 * c)) & 0xFF) * w3;
-        float s = ((a + b) + d) + e;
-        out |= (uint32_t)to_u8(roundf(s)) << (8 * c);
-    }
+    for (int c = 0; c < 4; c++)
+        out |= bilinear_channel(T, c) << (8 * c);
     return out;
 }
 
