@@ -173,9 +173,22 @@ struct Interp {
         ta += ts;
         ba += bs;
     }
+    // n single steps (the same adds in the same order), four per loop trip:
+    // a long replay then spends its time on the adds, not on loop control
     SW_HD void steps(int n)
     {
-        for (int i = 0; i < n; i++) {
+        int i = 0;
+        for (; i + 4 <= n; i += 4) {
+            ta += ts;
+            ba += bs;
+            ta += ts;
+            ba += bs;
+            ta += ts;
+            ba += bs;
+            ta += ts;
+            ba += bs;
+        }
+        for (; i < n; i++) {
             ta += ts;
             ba += bs;
         }
@@ -189,7 +202,16 @@ struct Side {
     float ratio, x;
     SW_HD void steps(int n)
     {
-        for (int i = 0; i < n; i++) {
+        int i = 0;
+        for (; i + 4 <= n; i += 4) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                x += ratio;
+                it.ta += it.ts;
+                it.ba += it.bs;
+            }
+        }
+        for (; i < n; i++) {
             x += ratio;
             it.ta += it.ts;
             it.ba += it.bs;
