@@ -1551,6 +1551,9 @@ __host__ __device__ inline size_t dof_lds_bytes(int R)
 // 8 contiguous runs, one per XCD: neighbouring tiles, whose halos overlap,
 // then share that XCD's L2.  The raster stored, per pixel, whether its blur
 // factor is nonzero (an "unfocused" tap) and its clamped radius.
+// RMAX bounds the wave's halo (the loads below are unrolled for it): the
+// launch picks 8 when no job of the launch has a larger radius.
+template <int RMAX>
 __global__ __launch_bounds__(256) void k_dof(BatchDev B)
 {
     extern __shared__ uint32_t s_dyn[];
@@ -1568,8 +1571,8 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B)
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     // Every load of the halo is issued before the first is used, so the
     // workgroup waits for memory once rather than once per row.
-    constexpr int ROWS = (SW_DOF_TH + 2 * SW_MAX_DOF_RADIUS + 3) / 4;
-    constexpr int COLS = (SW_DOF_TW + 2 * SW_MAX_DOF_RADIUS + 63) / 64;
+    constexpr int ROWS = (SW_DOF_TH + 2 * RMAX + 3) / 4;
+    constexpr int COLS = (SW_DOF_TW + 2 * RMAX + 63) / 64;
     uint32_t cv[ROWS][COLS];
     uint32_t kv[ROWS][COLS];
 #pragma unroll
@@ -1738,8 +1741,12 @@ hipError_t launch_layers(const BatchDev &B, bool generic, uint32_t pixels, hipSt
 }
 hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStream_t s)
 {
-    if (blocks)
-        LAUNCH_CHECK(k_dof, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B);
+    if (!blocks)
+        return hipSuccess;
+    if (max_radius <= 8)
+        LAUNCH_CHECK(k_dof<8>, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B);
+    else
+        LAUNCH_CHECK(k_dof<SW_MAX_DOF_RADIUS>, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B);
     return hipSuccess;
 }
 
