@@ -641,6 +641,19 @@ __device__ void vertex_attr(const JobDev &J, const NodeDev &N, const SVert &v, f
     n = normal_world(N.RS, nobj);
 }
 
+// The job's view of primitive P's texture (pixel_shaders.cpp:288-302).
+__device__ __forceinline__ TexView tex_view(const JobDev &J, const PrimDev &P)
+{
+    TexView X{nullptr, P.color, 1, 1};
+    if (P.tex >= 0) {
+        const TexDev T = J.texs[P.tex];
+        X.p = J.texels + T.offset;
+        X.w = T.w;
+        X.h = T.h;
+    }
+    return X;
+}
+
 // GENERIC: the job's shader is any SWEGL_SHADER_* kind, decoded into a light
 // part (0 none, 1 flat, 2 phong) and a texture part (0 the material colour,
 // 1 nearest, 2 bilinear); otherwise it is SWEGL_SHADER_PHONG_BILINEAR, the
@@ -661,15 +674,8 @@ __device__ uint32_t shade_from(const JobDev &J, uint32_t t, const Setup &S, cons
     }
     const PrimDev &P = J.prims[prim_of_tri(J, t)];
     const NodeDev &N = J.nodes[P.node];
-    uint32_t one = P.color;
-    const uint32_t *bm = &one;
-    int tw = 1, th = 1;
-    if (P.tex >= 0) {
-        TexDev T = J.texs[P.tex];
-        bm = J.texels + T.offset;
-        tw = T.w;
-        th = T.h;
-    }
+    const TexView X = tex_view(J, P);
+    const int tw = X.w, th = X.h;
     TriAttr A;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -682,7 +688,7 @@ __device__ uint32_t shade_from(const JobDev &J, uint32_t t, const Setup &S, cons
     }
     ScanShade SS = prepare_scanline(A, R.lower, R.lr, sp.pl, sp.pr);
     if (!GENERIC) {
-        uint32_t color = bilinear(bm, tw, th, SS.t_left + SS.t_dir * u, oob);
+        uint32_t color = bilinear(X, SS.t_left + SS.t_dir * u, oob);
         f3 center = SS.v + SS.vdir * u;
         f3 normal = normalize(SS.n + SS.ndir * u);
         int32_t light = phong(center, normal, J.V.cam, J.ambient, J.sun, J.sun_I, J.lights, (int)J.n_lights);
@@ -690,9 +696,9 @@ __device__ uint32_t shade_from(const JobDev &J, uint32_t t, const Setup &S, cons
     }
     uint32_t color = P.color;   // pixel_shader_t::color (default material for material_id -1)
     if (tk == 1)
-        color = nearest(bm, (uint32_t)tw, (uint32_t)th, SS.t_left + SS.t_dir * u);
+        color = nearest(X, SS.t_left + SS.t_dir * u);
     else if (tk == 2)
-        color = bilinear(bm, tw, th, SS.t_left + SS.t_dir * u, oob);
+        color = bilinear(X, SS.t_left + SS.t_dir * u, oob);
     if (lk == 0)
         return color;
     int32_t light;
@@ -724,15 +730,8 @@ __device__ uint32_t texture_from(const JobDev &J, uint32_t t, const Setup &S, co
     if (tk == 0)
         return P.color;
     const NodeDev &N = J.nodes[P.node];
-    uint32_t one = P.color;
-    const uint32_t *bm = &one;
-    int tw = 1, th = 1;
-    if (P.tex >= 0) {
-        TexDev T = J.texs[P.tex];
-        bm = J.texels + T.offset;
-        tw = T.w;
-        th = T.h;
-    }
+    const TexView X = tex_view(J, P);
+    const int tw = X.w, th = X.h;
     TriAttr A;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -745,8 +744,8 @@ __device__ uint32_t texture_from(const JobDev &J, uint32_t t, const Setup &S, co
     }
     ScanShade SS = prepare_scanline(A, R.lower, R.lr, sp.pl, sp.pr);
     if (tk == 1)
-        return nearest(bm, (uint32_t)tw, (uint32_t)th, SS.t_left + SS.t_dir * u);
-    return bilinear(bm, tw, th, SS.t_left + SS.t_dir * u, oob);
+        return nearest(X, SS.t_left + SS.t_dir * u);
+    return bilinear(X, SS.t_left + SS.t_dir * u, oob);
 }
 
 // Shading of the winning draw sequence seq at pixel (x, y): the sub-triangle,
@@ -1163,24 +1162,17 @@ __device__ uint64_t layer_walk(const JobDev &J, uint32_t t, const SubTri &st, in
         tk = (k == 1 || k == 3 || k == 7) ? 2 : (k == 2 || k == 4 || k == 6) ? 1 : 0;
     }
     const PrimDev &P = J.prims[prim_of_tri(J, t)];
-    uint32_t one = P.color;
-    const uint32_t *bm = &one;
-    int tw = 1, th = 1;
+    TexView X{nullptr, P.color, 1, 1};
     TriAttr A;
     uint64_t opq = 0ull;
     const bool textured = opaque != nullptr && tk != 0;
     if (textured) {
-        if (P.tex >= 0) {
-            TexDev T = J.texs[P.tex];
-            bm = J.texels + T.offset;
-            tw = T.w;
-            th = T.h;
-        }
+        X = tex_view(J, P);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             A.w[k] = f3{0.0f, 0.0f, 0.0f};
             A.n[k] = f3{0.0f, 0.0f, 0.0f};
-            A.t[k] = f2{J.tu[S.v[k].a] * (float)tw, J.tv[S.v[k].a] * (float)th};
+            A.t[k] = f2{J.tu[S.v[k].a] * (float)X.w, J.tv[S.v[k].a] * (float)X.h};
         }
     }
     const bool flat_opaque = (P.color >> 24) == 255;   // tk == 0: the material colour
@@ -1223,8 +1215,8 @@ __device__ uint64_t layer_walk(const JobDev &J, uint32_t t, const SubTri &st, in
                         if (textured) {
                             const f2 tc = SS.t_left + SS.t_dir * u;
                             uint32_t no_count = 0;   // texel reads are counted where the survivors are shaded
-                            op = tk == 1 ? (nearest(bm, (uint32_t)tw, (uint32_t)th, tc) >> 24) == 255
-                                         : bilinear_channel(bilinear_taps(bm, tw, th, tc, no_count), 3) == 255;
+                            op = tk == 1 ? (nearest(X, tc) >> 24) == 255
+                                         : bilinear_channel(bilinear_taps(X, tc, no_count), 3) == 255;
                         }
                         opq |= (uint64_t)op << p;
                     }

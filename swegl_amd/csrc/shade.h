@@ -85,14 +85,26 @@ SW_D int rem(int x, int m)
     return x % m;
 }
 
+// A texture as the shaders read it: w x h texels at p, or, with p null, the
+// 1x1 texture holding the material colour (pixel_shaders.cpp:288-302).
+// Keeping that colour by value (not behind a pointer to a local) keeps the
+// kernels free of private memory.
+struct TexView {
+    const uint32_t *p;
+    uint32_t one;
+    int w, h;
+};
+SW_D uint32_t texel(const TexView &X, int64_t i) { return X.p ? X.p[i] : X.one; }
+
 // The four texels and weights of one bilinear sample.
 struct BilTaps {
     uint32_t p[4];
     float w[4];
 };
 
-SW_D BilTaps bilinear_taps(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
+SW_D BilTaps bilinear_taps(const TexView &X, f2 t, uint32_t &oob)
 {
+    const int tw = X.w, th = X.h;
     float v = t.x;
     float u = t.y;
     // (float)((double)u - 0.5), float minus double literal stored to float,
@@ -125,7 +137,7 @@ SW_D BilTaps bilinear_taps(const uint32_t *pc, int tw, int th, f2 t, uint32_t &o
             oob++;
             i = i < 0 ? 0 : lim - 1;
         }
-        T.p[k] = pc[i];
+        T.p[k] = texel(X, i);
     }
     T.w[0] = (u - u1) * (v - v1);
     T.w[1] = (u - u1) * (v2 - v);
@@ -145,9 +157,9 @@ SW_D uint32_t bilinear_channel(const BilTaps &T, int c)
     return (uint32_t)to_u8(roundf(s));
 }
 
-SW_D uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
+SW_D uint32_t bilinear(const TexView &X, f2 t, uint32_t &oob)
 {
-    const BilTaps T = bilinear_taps(pc, tw, th, t, oob);
+    const BilTaps T = bilinear_taps(X, t, oob);
     uint32_t out = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++)
@@ -158,12 +170,13 @@ SW_D uint32_t bilinear(const uint32_t *pc, int tw, int th, f2 t, uint32_t &oob)
 // pixel_shader_texture::shade (pixel_shaders.cpp:275-281): twidth and
 // theight are unsigned there, so (int)t % twidth is the unsigned remainder
 // of the int's bits, always inside the bitmap; column from t.x, row from t.y.
-SW_D uint32_t nearest(const uint32_t *pc, uint32_t tw, uint32_t th, f2 t)
+SW_D uint32_t nearest(const TexView &X, f2 t)
 {
+    const uint32_t tw = (uint32_t)X.w, th = (uint32_t)X.h;
     const uint32_t x = (uint32_t)cvtt(t.x), y = (uint32_t)cvtt(t.y);
     const uint32_t u = (tw & (tw - 1)) == 0 ? x & (tw - 1) : x % tw;
     const uint32_t v = (th & (th - 1)) == 0 ? y & (th - 1) : y % th;
-    return pc[v * tw + u];
+    return texel(X, v * tw + u);
 }
 
 // pixel_shader_lights_flat::prepare_for_triangle (pixel_shaders.cpp:33-84),
