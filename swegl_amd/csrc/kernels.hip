@@ -70,8 +70,14 @@ __device__ __forceinline__ f3 ld3(const float4 *a, uint32_t i)
 // Wave-aggregated atomicAdd.  Lanes holding the same key are grouped with
 // ballots (ALU only); then every group leader issues its atomic in the same
 // instruction, so the wave waits for one round trip instead of one per key.
+// Grouping costs a lane exchange per distinct key, so it stops after
+// SW_WAVE_ADD_GROUPS groups: the lanes left (many distinct keys, as a large
+// triangle's tiles are) add 1 on their own in that same instruction.
 // Returns this lane's slot (old value + rank among same-key lanes) when
 // `want_slot`; inactive lanes get 0.
+#ifndef SW_WAVE_ADD_GROUPS
+#define SW_WAVE_ADD_GROUPS 4
+#endif
 template <bool want_slot>
 __device__ __forceinline__ uint32_t wave_add(uint32_t *base, uint32_t key, bool active)
 {
@@ -79,8 +85,8 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t *base, uint32_t key, bool 
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t pending = __ballot(active);
     int my_leader = lane;
-    uint32_t my_rank = 0, count = 0;
-    while (pending) {
+    uint32_t my_rank = 0, count = 1;   // a lane left ungrouped adds 1 itself
+    for (int g = 0; pending && g < SW_WAVE_ADD_GROUPS; g++) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lkey = __shfl(key, leader);
         const uint64_t same = __ballot(active && key == lkey);
