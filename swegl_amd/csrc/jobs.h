@@ -109,7 +109,7 @@ struct WorldArgs {
 };
 
 // Tile counters: tile g's bin count at tile_count[g * SW_TILE_PAD] and its
-// fill cursor one word later, a 128-byte line per tile, so the binning
+// fill rank (entries placed so far) one word later, a 128-byte line per tile, so the binning
 // atomics of different tiles spread over all memory channels instead of
 // queueing on the few that a dense array maps to.
 #define SW_TILE_PAD 32
@@ -120,7 +120,7 @@ struct BatchDev {
     uint32_t n_tiles;         // total tiles over all jobs
     uint32_t *tile_count;     // [n_tiles * SW_TILE_PAD], see SW_TILE_PAD
     uint32_t *tile_offset;    // [n_tiles + 1]
-    uint32_t *tile_cursor;    // tile_count + 1
+    uint32_t *tile_cursor;    // tile_count + 1: the fill ranks
     uint32_t *bins;           // [capacity]
     uint32_t bin_capacity;
     uint32_t big_tiles;       // per-triangle bin limit; larger ones go to the big list

@@ -144,13 +144,15 @@ __global__ __launch_bounds__(256) void k_world(WorldArgs W)
 // ------------------------------------------------------------ k_vertex
 // world_to_camera_or_frustum (vertex_shaders.hpp:35-52, 61-71) per viewport:
 // frustum coordinates and cleared visibility marks.
-// It also zeroes the tile counts and counter words [w0, w0 + n_words) that
-// the binning kernels after it accumulate into.
+// It also zeroes the tile counts and fill ranks and the counter words
+// [w0, w0 + n_words) that the binning kernels after it accumulate into.
 __global__ __launch_bounds__(256) void k_vertex(BatchDev B, uint32_t w0, uint32_t n_words)
 {
     const uint32_t gi = blockIdx.x * 256u + threadIdx.x, stride = gridDim.x * 256u;
-    for (uint32_t k = gi; k < B.n_tiles; k += stride)
+    for (uint32_t k = gi; k < B.n_tiles; k += stride) {
         B.tile_count[k * SW_TILE_PAD] = 0;
+        B.tile_cursor[k * SW_TILE_PAD] = 0;
+    }
     for (uint32_t k = gi; k < n_words; k += stride)
         B.ctr[w0 + k] = 0;
     const JobDev &J = B.jobs[find_job<JV>(B, blockIdx.x)];
@@ -436,7 +438,9 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
 // of 1024 x SCAN_PER tiles: each thread loads its SCAN_PER consecutive
 // counts at once (one memory round trip per pass), the block scans the
 // per-thread sums (shuffles within each wave, then the 16 wave totals), and
-// each thread writes its tiles' offsets and fill cursors.
+// each thread writes its tiles' offsets (densely: the fill ranks, zeroed by
+// k_vertex, sit in the padded lines and are added to the offsets by
+// k_bin_fill).
 #define SCAN_PER 8
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
@@ -484,7 +488,6 @@ __global__ __launch_bounds__(1024) void k_scan(BatchDev B)
         for (int k = 0; k < SCAN_PER; k++) {
             if (i0 + k < n) {
                 B.tile_offset[i0 + k] = run;
-                B.tile_cursor[(i0 + k) * SW_TILE_PAD] = run;
                 run += v[k];
             }
         }
@@ -524,7 +527,9 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B)
             const int w = tx1 - tx0 + 1;
             key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
         }
-        const uint32_t pos = wave_add<true>(B.tile_cursor, key * SW_TILE_PAD, act);
+        // the entry's place: the tile's offset plus its rank among the tile's entries so far
+        const uint32_t rank = wave_add<true>(B.tile_cursor, key * SW_TILE_PAD, act);
+        const uint32_t pos = act ? B.tile_offset[key] + rank : 0u;
         if (act && pos < B.bin_capacity)
             B.bins[pos] = t;
     }
