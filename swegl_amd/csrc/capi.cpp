@@ -647,6 +647,8 @@ static int scene_pose(swegl_ctx *c, swegl_scene *s, const swegl_scene_desc *d)
     W.nx = s->nx.as<float>();
     W.ny = s->ny.as<float>();
     W.nz = s->nz.as<float>();
+    W.tu = s->tu.as<float>();
+    W.tv = s->tv.as<float>();
     W.prims = s->prims.as<PrimDev>();
     W.nodes = s->nodes.as<NodeDev>();
     W.n_prims = s->n_prims;

@@ -10,7 +10,8 @@
 //   prims / nodes / texs         : small tables
 //   texels                       : uint32 bitmaps of all textures, concatenated
 //   world, nworld                : float4[n_vertices]  v_world / normal_world for the
-//                                                      current node poses
+//                                                      current node poses; .w holds the
+//                                                      texture coordinate u / v
 // Per viewport job:
 //   ndc                          : float4[n_vertices]  frustum coordinates
 //   yes                          : uint8[n_vertices]   visibility marks
@@ -101,7 +102,7 @@ struct JobDev {
 };
 
 struct WorldArgs {
-    const float *px, *py, *pz, *nx, *ny, *nz;
+    const float *px, *py, *pz, *nx, *ny, *nz, *tu, *tv;
     const PrimDev *prims;
     const NodeDev *nodes;
     uint32_t n_prims, n_vertices;

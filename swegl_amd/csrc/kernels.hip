@@ -137,8 +137,10 @@ __global__ __launch_bounds__(256) void k_world(WorldArgs W)
     const NodeDev &N = W.nodes[W.prims[lo].node];
     const f3 w = xform(N.M, f3{W.px[i], W.py[i], W.pz[i]});
     const f3 n = normal_world(N.RS, f3{W.nx[i], W.ny[i], W.nz[i]});
-    W.world[i] = make_float4(w.x, w.y, w.z, 0.0f);
-    W.nworld[i] = make_float4(n.x, n.y, n.z, 0.0f);
+    // the texture coordinates ride in the fourth lanes, so a shading lane
+    // gets a vertex's position, normal and coordinates in two loads
+    W.world[i] = make_float4(w.x, w.y, w.z, W.tu[i]);
+    W.nworld[i] = make_float4(n.x, n.y, n.z, W.tv[i]);
 }
 
 // ------------------------------------------------------------ k_vertex
@@ -634,18 +636,21 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
 }
 
 // Shading attributes of a sub-triangle vertex (v_world, normal_world, raw tex
-// coords); clip vertices are rebuilt as in renderer.cpp:290-297.
+// coords, the last in the fourth lanes of world and nworld); clip vertices
+// are rebuilt as in renderer.cpp:290-297.
 __device__ void vertex_attr(const JobDev &J, const NodeDev &N, const SVert &v, f3 &w, f3 &n, f2 &t)
 {
     if (!v.clip) {
-        w = ld3(J.world, v.a);
-        n = ld3(J.nworld, v.a);
-        t = f2{J.tu[v.a], J.tv[v.a]};
+        const float4 w4 = J.world[v.a], n4 = J.nworld[v.a];
+        w = f3{w4.x, w4.y, w4.z};
+        n = f3{n4.x, n4.y, n4.z};
+        t = f2{w4.w, n4.w};
         return;
     }
-    f3 wa = ld3(J.world, v.a), wb = ld3(J.world, v.b);
+    const float4 wa4 = J.world[v.a], wb4 = J.world[v.b];
+    const f3 wa = {wa4.x, wa4.y, wa4.z}, wb = {wb4.x, wb4.y, wb4.z};
     w = wa + (wb - wa) * v.cut;
-    f2 ta = {J.tu[v.a], J.tv[v.a]}, tb = {J.tu[v.b], J.tv[v.b]};
+    f2 ta = {wa4.w, J.nworld[v.a].w}, tb = {wb4.w, J.nworld[v.b].w};
     t = ta + (tb - ta) * v.cut;
     f3 na = {J.nx[v.a], J.ny[v.a], J.nz[v.a]}, nb = {J.nx[v.b], J.ny[v.b], J.nz[v.b]};
     f3 nobj = (na.x == nb.x && na.y == nb.y && na.z == nb.z) ? na : normalize(na + (nb - na) * v.cut);
@@ -1183,7 +1188,7 @@ __device__ uint64_t layer_walk(const JobDev &J, uint32_t t, const SubTri &st, in
         for (int k = 0; k < 3; k++) {
             A.w[k] = f3{0.0f, 0.0f, 0.0f};
             A.n[k] = f3{0.0f, 0.0f, 0.0f};
-            A.t[k] = f2{J.tu[S.v[k].a] * (float)X.w, J.tv[S.v[k].a] * (float)X.h};
+            A.t[k] = f2{J.world[S.v[k].a].w * (float)X.w, J.nworld[S.v[k].a].w * (float)X.h};
         }
     }
     const bool flat_opaque = (P.color >> 24) == 255;   // tk == 0: the material colour
