@@ -213,10 +213,14 @@ __device__ SVert clip_vertex(const JobDev &J, uint32_t a, uint32_t b, float cut)
 
 // fill_triangle (renderer.cpp:240-359): the sub-triangles handed to
 // fill_triangle_2, in call order.  Returns how many (0, 1 or 2).
+// BINNED: t came out of the binning, which lists a triangle only if all
+// three of its vertices are marked (quick_extent, and this function with
+// marks checked on the exact path), so the marks need no second look.
+template <bool BINNED = false>
 __device__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
 {
     uint32_t i0 = J.tri[3 * t], i1 = J.tri[3 * t + 1], i2 = J.tri[3 * t + 2];
-    if (!(J.yes[i0] && J.yes[i1] && J.yes[i2]))
+    if (!BINNED && !(J.yes[i0] && J.yes[i1] && J.yes[i2]))
         return 0;
     SVert v0 = orig_vertex(J, i0), v1 = orig_vertex(J, i1), v2 = orig_vertex(J, i2);
     bool front = cross_z_sign(v1.s - v0.s, v2.s - v0.s, -1);
@@ -576,7 +580,7 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
                              unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans)
 {
     SubTri st[2];
-    int ns = tri_subs(J, t, st);
+    int ns = tri_subs<true>(J, t, st);
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         Setup S;
@@ -776,7 +780,7 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
     Span sp;
     // The winner was produced by this very replay, so every step below
     // succeeds; the guards only keep a corrupted key from looping.
-    if ((int)s >= tri_subs(J, t, st) || !setup_subtri(s ? st[1] : st[0], J.V, S, y) || !row_state(S, y, R) ||
+    if ((int)s >= tri_subs<true>(J, t, st) || !setup_subtri(s ? st[1] : st[0], J.V, S, y) || !row_state(S, y, R) ||
         !span_of(R, J.V, sp) || x < sp.x1 || x >= sp.x2)
         return 0u;
     Interp q = sp.q;
@@ -1265,7 +1269,7 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
     if (threadIdx.x < n) {
         const uint32_t t = list[threadIdx.x];
         SubTri st[2];
-        const int ns = tri_subs(J, t, st);
+        const int ns = tri_subs<true>(J, t, st);
         LayerSub &L = s_sub[threadIdx.x];
         uint64_t m = 0ull;
         if (ns == 1 && !st[0].v[0].clip && !st[0].v[1].clip && !st[0].v[2].clip) {
@@ -1318,7 +1322,7 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
             });
         } else if (L.kind == 4u) {
             SubTri st[2];
-            const int ns = tri_subs(J, tt, st);
+            const int ns = tri_subs<true>(J, tt, st);
             for (int sub = 0; sub < ns; sub++) {
                 Setup S;
                 Row R;
