@@ -789,10 +789,11 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
 }
 
 template <int TW, int TH, bool GENERIC>
-__device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *keys, uint32_t *s_cnt)
+__device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__restrict__ jobs, uint32_t g,
+                                            unsigned long long *keys, uint32_t *s_cnt)
 {
     const int jidx = find_job<JTILE>(B, g);
-    const JobDev &J = B.jobs[jidx];
+    const JobDev &J = jobs[jidx];
     const int lt = (int)(g - J.tile_base);
     const int tx = lt % J.tiles_x, ty = lt / J.tiles_x;
     TileRect tr;
@@ -953,7 +954,8 @@ __device__ void raster_tile(const BatchDev &B, uint32_t g, unsigned long long *k
 // At most 128 VGPRs, so four waves fit on each SIMD (512 registers a lane);
 // the shading path needs about that many and one more halves the overlap.
 template <int TW, int TH, bool GENERIC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_raster(BatchDev B)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_raster(BatchDev B,
+                                                                                     const JobDev *__restrict__ jobs)
 {
     __shared__ unsigned long long keys[TW * TH];
     __shared__ uint32_t s_cnt[3];
@@ -994,7 +996,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const uint32_t g = s_tile;
         if (g == 0xFFFFFFFFu)
             break;
-        raster_tile<TW, TH, GENERIC>(B, g, keys, s_cnt);
+        raster_tile<TW, TH, GENERIC>(B, jobs, g, keys, s_cnt);
         __syncthreads();   // tile done with keys and s_tile before the next claim
     }
     __syncthreads();
@@ -1714,17 +1716,17 @@ static hipError_t launch_raster_t(const BatchDev &B, int tw, int th, dim3 g, hip
 {
     dim3 b(256);
     if (tw == 8 && th == 8)
-        LAUNCH_CHECK((k_raster<8, 8, G>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<8, 8, G>), g, b, 0, s, B, B.jobs);
     else if (tw == 16 && th == 16)
-        LAUNCH_CHECK((k_raster<16, 16, G>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<16, 16, G>), g, b, 0, s, B, B.jobs);
     else if (tw == 32 && th == 16)
-        LAUNCH_CHECK((k_raster<32, 16, G>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<32, 16, G>), g, b, 0, s, B, B.jobs);
     else if (tw == 32 && th == 32)
-        LAUNCH_CHECK((k_raster<32, 32, G>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<32, 32, G>), g, b, 0, s, B, B.jobs);
     else if (tw == 64 && th == 32)
-        LAUNCH_CHECK((k_raster<64, 32, G>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<64, 32, G>), g, b, 0, s, B, B.jobs);
     else if (tw == 64 && th == 64)
-        LAUNCH_CHECK((k_raster<64, 64, G>), g, b, 0, s, B);
+        LAUNCH_CHECK((k_raster<64, 64, G>), g, b, 0, s, B, B.jobs);
     else
         return hipErrorInvalidValue;
     return hipSuccess;
