@@ -23,13 +23,16 @@ __device__ __forceinline__ uint32_t first_block(const JobDev &j)
 {
     return F == JV ? j.vblock0 : F == JT ? j.tblock0 : F == JD ? j.dblock0 : j.tile_base;
 }
+// Every kernel takes the job table as a `const __restrict__` argument of its
+// own as well: the table is not written while kernels run, and through such
+// an argument the compiler reads uniform job fields with scalar loads.
 template <int F>
-__device__ __forceinline__ int find_job(const BatchDev &B, uint32_t blk)
+__device__ __forceinline__ int find_job(const BatchDev &B, const JobDev *__restrict__ jobs, uint32_t blk)
 {
     int lo = 0, hi = (int)B.n_jobs - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (first_block<F>(B.jobs[mid]) <= blk)
+        if (first_block<F>(jobs[mid]) <= blk)
             lo = mid;
         else
             hi = mid - 1;
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(256) void k_world(WorldArgs W)
 // frustum coordinates and cleared visibility marks.
 // It also zeroes the tile counts and fill ranks and the counter words
 // [w0, w0 + n_words) that the binning kernels after it accumulate into.
-__global__ __launch_bounds__(256) void k_vertex(BatchDev B, uint32_t w0, uint32_t n_words)
+__global__ __launch_bounds__(256) void k_vertex(BatchDev B, uint32_t w0, uint32_t n_words, const JobDev *__restrict__ jobs)
 {
     const uint32_t gi = blockIdx.x * 256u + threadIdx.x, stride = gridDim.x * 256u;
     for (uint32_t k = gi; k < B.n_tiles; k += stride) {
@@ -157,7 +160,7 @@ __global__ __launch_bounds__(256) void k_vertex(BatchDev B, uint32_t w0, uint32_
     }
     for (uint32_t k = gi; k < n_words; k += stride)
         B.ctr[w0 + k] = 0;
-    const JobDev &J = B.jobs[find_job<JV>(B, blockIdx.x)];
+    const JobDev &J = jobs[find_job<JV>(B, jobs, blockIdx.x)];
     const uint32_t i = (blockIdx.x - J.vblock0) * 256u + threadIdx.x;
     if (i >= J.n_vertices)
         return;
@@ -170,9 +173,9 @@ __global__ __launch_bounds__(256) void k_vertex(BatchDev B, uint32_t w0, uint32_
 // -------------------------------------------------------------- k_mark
 // renderer.cpp:83-185: a vertex is marked when any triangle using it passes
 // the frustum test and (double-sided or front-facing) in frustum space.
-__global__ __launch_bounds__(256) void k_mark(BatchDev B)
+__global__ __launch_bounds__(256) void k_mark(BatchDev B, const JobDev *__restrict__ jobs)
 {
-    const JobDev &J = B.jobs[find_job<JT>(B, blockIdx.x)];
+    const JobDev &J = jobs[find_job<JT>(B, jobs, blockIdx.x)];
     uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
     if (t >= J.n_tris)
         return;
@@ -348,10 +351,10 @@ __device__ bool quick_extent(const JobDev &J, uint32_t t, Extent &E, uint32_t &d
     return E.x0 <= E.x1;
 }
 
-__global__ __launch_bounds__(256) void k_bin_count(BatchDev B)
+__global__ __launch_bounds__(256) void k_bin_count(BatchDev B, const JobDev *__restrict__ jobs)
 {
-    const int jidx = find_job<JT>(B, blockIdx.x);
-    const JobDev &J = B.jobs[jidx];
+    const int jidx = find_job<JT>(B, jobs, blockIdx.x);
+    const JobDev &J = jobs[jidx];
     const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
     uint32_t drawn = 0, big = 0;
     int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
@@ -509,9 +512,9 @@ __global__ __launch_bounds__(1024) void k_scan(BatchDev B)
 }
 
 // ----------------------------------------------------------- k_bin_fill
-__global__ __launch_bounds__(256) void k_bin_fill(BatchDev B)
+__global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__restrict__ jobs)
 {
-    const JobDev &J = B.jobs[find_job<JT>(B, blockIdx.x)];
+    const JobDev &J = jobs[find_job<JT>(B, jobs, blockIdx.x)];
     const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
     int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
     if (t < J.n_tris) {
@@ -792,7 +795,7 @@ template <int TW, int TH, bool GENERIC>
 __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__restrict__ jobs, uint32_t g,
                                             unsigned long long *keys, uint32_t *s_cnt)
 {
-    const int jidx = find_job<JTILE>(B, g);
+    const int jidx = find_job<JTILE>(B, jobs, g);
     const JobDev &J = jobs[jidx];
     const int lt = (int)(g - J.tile_base);
     const int tx = lt % J.tiles_x, ty = lt / J.tiles_x;
@@ -1347,7 +1350,7 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
 }
 
 template <bool GENERIC>
-__global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
+__global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B, const JobDev *__restrict__ jobs)
 {
     constexpr int TW = SW_LAYER_TW, TH = SW_LAYER_TH, KMAX = SW_MAX_LAYERS, NT = SW_LAYER_THREADS;
     static_assert(TW * TH == 64 && NT == 64, "the layer raster works on 64-pixel tiles, one wave each");
@@ -1357,8 +1360,8 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
     __shared__ float s_z[CH * 64];
     __shared__ uint64_t s_mask[CH], s_opq[CH];
     const uint32_t g = blockIdx.x;
-    const int jidx = find_job<JTILE>(B, g);
-    const JobDev &J = B.jobs[jidx];
+    const int jidx = find_job<JTILE>(B, jobs, g);
+    const JobDev &J = jobs[jidx];
     const int lt = (int)(g - J.tile_base);
     const int tx = lt % J.tiles_x, ty = lt / J.tiles_x;
     const int tx0 = J.V.vx + tx * TW, ty0 = J.V.vy + ty * TH;
@@ -1507,9 +1510,9 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B)
 // viewport_t::flatten(layer 0) (viewport.cpp:53-67): layer 0 over the screen
 // pixel at (i, j) from the surface's origin -- the reference ignores the
 // viewport's offset here -- or the screen pixel where layer 0 is empty.
-__global__ __launch_bounds__(256) void k_tflatten(BatchDev B, uint32_t job)
+__global__ __launch_bounds__(256) void k_tflatten(BatchDev B, uint32_t job, const JobDev *__restrict__ jobs)
 {
-    const JobDev &J = B.jobs[job];
+    const JobDev &J = jobs[job];
     const uint32_t p = blockIdx.x * 256u + threadIdx.x;
     if (p >= (uint32_t)(J.V.vw * J.V.vh))
         return;
@@ -1520,9 +1523,9 @@ __global__ __launch_bounds__(256) void k_tflatten(BatchDev B, uint32_t job)
 }
 // post_shader_t::shade with layers (post_shaders.hpp:22-47): layer 0 to the
 // screen rows y + j, columns from 0 (the reference ignores the x offset).
-__global__ __launch_bounds__(256) void k_tcopy(BatchDev B, uint32_t job)
+__global__ __launch_bounds__(256) void k_tcopy(BatchDev B, uint32_t job, const JobDev *__restrict__ jobs)
 {
-    const JobDev &J = B.jobs[job];
+    const JobDev &J = jobs[job];
     const uint32_t p = blockIdx.x * 256u + threadIdx.x;
     if (p >= (uint32_t)(J.V.vw * J.V.vh))
         return;
@@ -1568,10 +1571,10 @@ __host__ __device__ inline size_t dof_lds_bytes(int R)
 // RMAX bounds the wave's halo (the loads below are unrolled for it): the
 // launch picks 8 when no job of the launch has a larger radius.
 template <int RMAX>
-__global__ __launch_bounds__(256) void k_dof(BatchDev B)
+__global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restrict__ jobs)
 {
     extern __shared__ uint32_t s_dyn[];
-    const JobDev &J = B.jobs[find_job<JD>(B, blockIdx.x)];
+    const JobDev &J = jobs[find_job<JD>(B, jobs, blockIdx.x)];
     const uint32_t n = dof_blocks(J.V.vw, J.V.vh), q = n / 8;
     uint32_t lb = blockIdx.x - J.dblock0;
     if (lb < 8 * q)
@@ -1685,19 +1688,19 @@ hipError_t launch_world(const WorldArgs &W, hipStream_t s)
 hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, uint32_t w0, uint32_t n_words, hipStream_t s)
 {
     // at least one block: the kernel also zeroes the counters
-    LAUNCH_CHECK(k_vertex, dim3(blocks ? blocks : 1), dim3(256), 0, s, B, w0, n_words);
+    LAUNCH_CHECK(k_vertex, dim3(blocks ? blocks : 1), dim3(256), 0, s, B, w0, n_words, B.jobs);
     return hipSuccess;
 }
 hipError_t launch_mark(const BatchDev &B, uint32_t blocks, hipStream_t s)
 {
     if (blocks)
-        LAUNCH_CHECK(k_mark, dim3(blocks), dim3(256), 0, s, B);
+        LAUNCH_CHECK(k_mark, dim3(blocks), dim3(256), 0, s, B, B.jobs);
     return hipSuccess;
 }
 hipError_t launch_bin_count(const BatchDev &B, uint32_t blocks, hipStream_t s)
 {
     if (blocks)
-        LAUNCH_CHECK(k_bin_count, dim3(blocks), dim3(256), 0, s, B);
+        LAUNCH_CHECK(k_bin_count, dim3(blocks), dim3(256), 0, s, B, B.jobs);
     return hipSuccess;
 }
 hipError_t launch_scan(const BatchDev &B, hipStream_t s)
@@ -1708,7 +1711,7 @@ hipError_t launch_scan(const BatchDev &B, hipStream_t s)
 hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s)
 {
     if (blocks)
-        LAUNCH_CHECK(k_bin_fill, dim3(blocks), dim3(256), 0, s, B);
+        LAUNCH_CHECK(k_bin_fill, dim3(blocks), dim3(256), 0, s, B, B.jobs);
     return hipSuccess;
 }
 template <bool G>
@@ -1745,12 +1748,12 @@ hipError_t launch_layers(const BatchDev &B, bool generic, uint32_t pixels, hipSt
         return hipSuccess;
     // one block per layer tile
     if (generic)
-        LAUNCH_CHECK(k_raster_layers<true>, dim3(B.n_tiles), dim3(SW_LAYER_THREADS), 0, s, B);
+        LAUNCH_CHECK(k_raster_layers<true>, dim3(B.n_tiles), dim3(SW_LAYER_THREADS), 0, s, B, B.jobs);
     else
-        LAUNCH_CHECK(k_raster_layers<false>, dim3(B.n_tiles), dim3(SW_LAYER_THREADS), 0, s, B);
+        LAUNCH_CHECK(k_raster_layers<false>, dim3(B.n_tiles), dim3(SW_LAYER_THREADS), 0, s, B, B.jobs);
     const uint32_t blocks = (pixels + 255) / 256;
-    LAUNCH_CHECK(k_tflatten, dim3(blocks), dim3(256), 0, s, B, 0u);
-    LAUNCH_CHECK(k_tcopy, dim3(blocks), dim3(256), 0, s, B, 0u);
+    LAUNCH_CHECK(k_tflatten, dim3(blocks), dim3(256), 0, s, B, 0u, B.jobs);
+    LAUNCH_CHECK(k_tcopy, dim3(blocks), dim3(256), 0, s, B, 0u, B.jobs);
     return hipSuccess;
 }
 hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStream_t s)
@@ -1758,9 +1761,9 @@ hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStr
     if (!blocks)
         return hipSuccess;
     if (max_radius <= 8)
-        LAUNCH_CHECK(k_dof<8>, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B);
+        LAUNCH_CHECK(k_dof<8>, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B, B.jobs);
     else
-        LAUNCH_CHECK(k_dof<SW_MAX_DOF_RADIUS>, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B);
+        LAUNCH_CHECK(k_dof<SW_MAX_DOF_RADIUS>, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B, B.jobs);
     return hipSuccess;
 }
 
