@@ -15,6 +15,33 @@
 
 namespace sw {
 
+// Arrays a kernel only reads (the scene, the frustum coordinates, marks, bins
+// and rectangles written by earlier kernels) are read through the constant
+// address space: uniform addresses then take scalar loads, divergent ones
+// global loads with a scalar base.  Never for an array the same kernel
+// writes.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SW_RO __attribute__((address_space(4)))
+#else
+#define SW_RO
+#endif
+template <class T>
+SW_HD const SW_RO T *ro(const T *p)
+{
+    return (const SW_RO T *)p;
+}
+// HIP's vector types cannot be copied out of another address space.
+SW_HD float4 ro4(const float4 *p, uint32_t i)
+{
+    const SW_RO float *q = ro((const float *)(p + i));
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+SW_HD uint2 ro2(const uint2 *p, uint32_t i)
+{
+    const SW_RO uint32_t *q = ro((const uint32_t *)(p + i));
+    return make_uint2(q[0], q[1]);
+}
+
 // ------------------------------------------------------------------ scalars
 SW_HD int32_t cvtt(float f)
 {

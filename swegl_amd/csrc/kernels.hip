@@ -44,7 +44,7 @@ __device__ __forceinline__ int prim_of_vertex(const JobDev &J, uint32_t v)
     int lo = 0, hi = (int)J.n_prims - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (J.prims[mid].first_vertex <= v)
+        if (ro(J.prims)[mid].first_vertex <= v)
             lo = mid;
         else
             hi = mid - 1;
@@ -56,7 +56,7 @@ __device__ __forceinline__ int prim_of_tri(const JobDev &J, uint32_t t)
     int lo = 0, hi = (int)J.n_prims - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (J.prims[mid].first_tri <= t)
+        if (ro(J.prims)[mid].first_tri <= t)
             lo = mid;
         else
             hi = mid - 1;
@@ -66,7 +66,7 @@ __device__ __forceinline__ int prim_of_tri(const JobDev &J, uint32_t t)
 
 __device__ __forceinline__ f3 ld3(const float4 *a, uint32_t i)
 {
-    float4 v = a[i];
+    const float4 v = ro4(a, i);
     return {v.x, v.y, v.z};
 }
 
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_vertex(BatchDev B, uint32_t w0, uint32_
     const uint32_t i = (blockIdx.x - J.vblock0) * 256u + threadIdx.x;
     if (i >= J.n_vertices)
         return;
-    const float4 w4 = J.world[i];
+    const float4 w4 = ro4(J.world, i);
     const f3 f = world_to_frustum(J.V, f3{w4.x, w4.y, w4.z});
     J.ndc[i] = make_float4(f.x, f.y, f.z, 0.0f);
     J.yes[i] = 0;
@@ -179,11 +179,11 @@ __global__ __launch_bounds__(256) void k_mark(BatchDev B, const JobDev *__restri
     uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
     if (t >= J.n_tris)
         return;
-    uint32_t i0 = J.tri[3 * t], i1 = J.tri[3 * t + 1], i2 = J.tri[3 * t + 2];
+    uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
     f3 a = ld3(J.ndc, i0), b = ld3(J.ndc, i1), c = ld3(J.ndc, i2);
     if (!inside_frustum(a, b, c))
         return;
-    if (!J.prims[prim_of_tri(J, t)].double_sided && !cross_z_sign(b - a, c - a, 1))
+    if (!ro(J.prims)[prim_of_tri(J, t)].double_sided && !cross_z_sign(b - a, c - a, 1))
         return;
     J.yes[i0] = 1;
     J.yes[i1] = 1;
@@ -222,8 +222,8 @@ __device__ SVert clip_vertex(const JobDev &J, uint32_t a, uint32_t b, float cut)
 template <bool BINNED = false>
 __device__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
 {
-    uint32_t i0 = J.tri[3 * t], i1 = J.tri[3 * t + 1], i2 = J.tri[3 * t + 2];
-    if (!BINNED && !(J.yes[i0] && J.yes[i1] && J.yes[i2]))
+    uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
+    if (!BINNED && !(ro(J.yes)[i0] && ro(J.yes)[i1] && ro(J.yes)[i2]))
         return 0;
     SVert v0 = orig_vertex(J, i0), v1 = orig_vertex(J, i1), v2 = orig_vertex(J, i2);
     bool front = cross_z_sign(v1.s - v0.s, v2.s - v0.s, -1);
@@ -319,8 +319,8 @@ __device__ bool sub_extent(const Setup &S, const ViewXf &V, Extent &E)
 __device__ bool quick_extent(const JobDev &J, uint32_t t, Extent &E, uint32_t &drawn, bool &exact)
 {
     exact = false;
-    uint32_t i0 = J.tri[3 * t], i1 = J.tri[3 * t + 1], i2 = J.tri[3 * t + 2];
-    if (!(J.yes[i0] && J.yes[i1] && J.yes[i2]))
+    uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
+    if (!(ro(J.yes)[i0] && ro(J.yes)[i1] && ro(J.yes)[i2]))
         return false;
     f3 a = frustum_to_screen(J.V, ld3(J.ndc, i0));
     f3 b = frustum_to_screen(J.V, ld3(J.ndc, i1));
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__re
     const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
     int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
     if (t < J.n_tris) {
-        const uint2 r = J.rect[t];
+        const uint2 r = ro2(J.rect, t);
         if (r.x != RECT_EMPTY) {
             tx0 = r.x & 0xFFFF;
             ty0 = r.x >> 16;
@@ -648,18 +648,18 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
 __device__ void vertex_attr(const JobDev &J, const NodeDev &N, const SVert &v, f3 &w, f3 &n, f2 &t)
 {
     if (!v.clip) {
-        const float4 w4 = J.world[v.a], n4 = J.nworld[v.a];
+        const float4 w4 = ro4(J.world, v.a), n4 = ro4(J.nworld, v.a);
         w = f3{w4.x, w4.y, w4.z};
         n = f3{n4.x, n4.y, n4.z};
         t = f2{w4.w, n4.w};
         return;
     }
-    const float4 wa4 = J.world[v.a], wb4 = J.world[v.b];
+    const float4 wa4 = ro4(J.world, v.a), wb4 = ro4(J.world, v.b);
     const f3 wa = {wa4.x, wa4.y, wa4.z}, wb = {wb4.x, wb4.y, wb4.z};
     w = wa + (wb - wa) * v.cut;
-    f2 ta = {wa4.w, J.nworld[v.a].w}, tb = {wb4.w, J.nworld[v.b].w};
+    f2 ta = {wa4.w, ro4(J.nworld, v.a).w}, tb = {wb4.w, ro4(J.nworld, v.b).w};
     t = ta + (tb - ta) * v.cut;
-    f3 na = {J.nx[v.a], J.ny[v.a], J.nz[v.a]}, nb = {J.nx[v.b], J.ny[v.b], J.nz[v.b]};
+    f3 na = {ro(J.nx)[v.a], ro(J.ny)[v.a], ro(J.nz)[v.a]}, nb = {ro(J.nx)[v.b], ro(J.ny)[v.b], ro(J.nz)[v.b]};
     f3 nobj = (na.x == nb.x && na.y == nb.y && na.z == nb.z) ? na : normalize(na + (nb - na) * v.cut);
     n = normal_world(N.RS, nobj);
 }
@@ -669,7 +669,7 @@ __device__ __forceinline__ TexView tex_view(const JobDev &J, const PrimDev &P)
 {
     TexView X{nullptr, P.color, 1, 1};
     if (P.tex >= 0) {
-        const TexDev T = J.texs[P.tex];
+        const TexDev T = ro(J.texs)[P.tex];
         X.p = J.texels + T.offset;
         X.w = T.w;
         X.h = T.h;
@@ -695,8 +695,8 @@ __device__ uint32_t shade_from(const JobDev &J, uint32_t t, const Setup &S, cons
         tk = (k == 1 || k == 3 || k == 7) ? 2 : (k == 2 || k == 4 || k == 6) ? 1 : 0;
         combined = k >= 1 && k <= 4;
     }
-    const PrimDev &P = J.prims[prim_of_tri(J, t)];
-    const NodeDev &N = J.nodes[P.node];
+    const PrimDev P = ro(J.prims)[prim_of_tri(J, t)];
+    const NodeDev N = ro(J.nodes)[P.node];
     const TexView X = tex_view(J, P);
     const int tw = X.w, th = X.h;
     TriAttr A;
@@ -749,10 +749,10 @@ __device__ uint32_t texture_from(const JobDev &J, uint32_t t, const Setup &S, co
         const int k = J.shader;
         tk = (k == 1 || k == 3 || k == 7) ? 2 : (k == 2 || k == 4 || k == 6) ? 1 : 0;
     }
-    const PrimDev &P = J.prims[prim_of_tri(J, t)];
+    const PrimDev P = ro(J.prims)[prim_of_tri(J, t)];
     if (tk == 0)
         return P.color;
-    const NodeDev &N = J.nodes[P.node];
+    const NodeDev N = ro(J.nodes)[P.node];
     const TexView X = tex_view(J, P);
     const int tw = X.w, th = X.h;
     TriAttr A;
@@ -825,10 +825,10 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
         for (uint32_t e = threadIdx.x; e < nb + nbig; e += 256) {
             uint32_t t;
             if (e < nb) {
-                t = B.bins[off + e];
+                t = ro(B.bins)[off + e];
             } else {
-                t = J.big[e - nb];
-                const uint2 r = J.rect[t];
+                t = ro(J.big)[e - nb];
+                const uint2 r = ro2(J.rect, t);
                 if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
                     ty > (int)(r.y >> 16))
                     continue;
@@ -842,10 +842,10 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
         int ry0 = tr.y0, ry1 = tr.y1;
         if (e < nb + nbig) {
             if (e < nb) {
-                t = B.bins[off + e];
+                t = ro(B.bins)[off + e];
             } else {
-                t = J.big[e - nb];
-                const uint2 r = J.rect[t];
+                t = ro(J.big)[e - nb];
+                const uint2 r = ro2(J.rect, t);
                 if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
                     ty > (int)(r.y >> 16))
                     t = 0xFFFFFFFFu;
@@ -854,7 +854,7 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
         if (t != 0xFFFFFFFFu) {
             // rows [ceil(ymin), ceil(ymax)) are exactly the rows fill_triangle_2 visits for an
             // unclipped triangle; clipped or non-finite ones keep all rows of the tile
-            const uint32_t i0 = J.tri[3 * t], i1 = J.tri[3 * t + 1], i2 = J.tri[3 * t + 2];
+            const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
             const f3 a = frustum_to_screen(J.V, ld3(J.ndc, i0)), b = frustum_to_screen(J.V, ld3(J.ndc, i1)),
                      c = frustum_to_screen(J.V, ld3(J.ndc, i2));
             const float big = 1073741824.0f;
@@ -1186,7 +1186,7 @@ __device__ uint64_t layer_walk(const JobDev &J, uint32_t t, const SubTri &st, in
         const int k = J.shader;
         tk = (k == 1 || k == 3 || k == 7) ? 2 : (k == 2 || k == 4 || k == 6) ? 1 : 0;
     }
-    const PrimDev &P = J.prims[prim_of_tri(J, t)];
+    const PrimDev P = ro(J.prims)[prim_of_tri(J, t)];
     TexView X{nullptr, P.color, 1, 1};
     TriAttr A;
     uint64_t opq = 0ull;
@@ -1197,7 +1197,7 @@ __device__ uint64_t layer_walk(const JobDev &J, uint32_t t, const SubTri &st, in
         for (int k = 0; k < 3; k++) {
             A.w[k] = f3{0.0f, 0.0f, 0.0f};
             A.n[k] = f3{0.0f, 0.0f, 0.0f};
-            A.t[k] = f2{J.world[S.v[k].a].w * (float)X.w, J.nworld[S.v[k].a].w * (float)X.h};
+            A.t[k] = f2{ro4(J.world, S.v[k].a).w * (float)X.w, ro4(J.nworld, S.v[k].a).w * (float)X.h};
         }
     }
     const bool flat_opaque = (P.color >> 24) == 255;   // tk == 0: the material colour
@@ -1386,7 +1386,7 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B, 
     const int lane = __lane_id();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     auto covers = [&](uint32_t t) {
-        const uint2 r = J.rect[t];
+        const uint2 r = ro2(J.rect, t);
         return r.x != RECT_EMPTY && tx >= (int)(r.x & 0xFFFF) && tx <= (int)(r.y & 0xFFFF) &&
                ty >= (int)(r.x >> 16) && ty <= (int)(r.y >> 16);
     };
@@ -1398,12 +1398,12 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B, 
     uint32_t total = 0;
     if (listed) {
         for (uint32_t e = threadIdx.x; e < nb; e += NT)
-            s_list[e] = B.bins[off + e];
+            s_list[e] = ro(B.bins)[off + e];
         total = nb;
         const int wave = (int)(threadIdx.x >> 6);
         for (uint32_t base = 0; base < nbig && listed; base += NT) {
             const uint32_t e = base + threadIdx.x;
-            const uint32_t t = e < nbig ? J.big[e] : 0u;
+            const uint32_t t = e < nbig ? ro(J.big)[e] : 0u;
             const bool cov = e < nbig && covers(t);
             const uint64_t bal = __ballot(cov);
             if (lane == 0)

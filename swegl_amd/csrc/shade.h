@@ -50,7 +50,7 @@ SW_D int32_t phong(f3 center, f3 normal, f3 cam, float ambient, f3 sun, float su
         fs *= sun_I;
     float acc = 0.0f;
     for (int i = 0; i < n_lights; i++) {
-        Light4 l = lights[i];
+        const Light4 l = ro(lights)[i];
         f3 L = center - f3{l.x, l.y, l.z};
         float d2 = L.x * L.x + L.y * L.y + L.z * L.z;
         float diffuse = l.I / d2;
@@ -94,7 +94,7 @@ struct TexView {
     uint32_t one;
     int w, h;
 };
-SW_D uint32_t texel(const TexView &X, int64_t i) { return X.p ? X.p[i] : X.one; }
+SW_D uint32_t texel(const TexView &X, int64_t i) { return X.p ? ro(X.p)[i] : X.one; }
 
 // The four texels and weights of one bilinear sample.
 struct BilTaps {
