@@ -1604,8 +1604,8 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restric
             cv[k][m] = 0u;
             kv[k][m] = 0u;
             if (row_ok && c < SW && lx >= 0 && lx < J.V.vw) {
-                cv[k][m] = crow[lx];
-                kv[k][m] = krow[lx];
+                cv[k][m] = ro(crow)[lx];   // the raster's output: read-only here
+                kv[k][m] = ro(krow)[lx];
             }
         }
     }
