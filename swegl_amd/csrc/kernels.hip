@@ -536,9 +536,11 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__re
             const int w = tx1 - tx0 + 1;
             key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
         }
-        // the entry's place: the tile's offset plus its rank among the tile's entries so far
+        // the entry's place: the tile's offset (read while the rank's atomic is
+        // in flight) plus its rank among the tile's entries so far
+        const uint32_t off = act ? ro(B.tile_offset)[key] : 0u;
         const uint32_t rank = wave_add<true>(B.tile_cursor, key * SW_TILE_PAD, act);
-        const uint32_t pos = act ? B.tile_offset[key] + rank : 0u;
+        const uint32_t pos = off + rank;
         if (act && pos < B.bin_capacity)
             B.bins[pos] = t;
     }
