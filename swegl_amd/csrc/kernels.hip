@@ -1636,14 +1636,22 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restric
         const int radius = s_code[c] >> 1;
         uint32_t out = src;
         if (radius != 0) {
+            // With radius <= 8 a sum has at most 256 taps of 255, under 2^16, so
+            // blue and red add as the two halves of one word.
+            constexpr bool PACKED = RMAX <= 8;
             uint32_t b = 0, g = 0, r = 0, count = 0;
             auto tap = [&](int k) {   // LDS cell k; adds it when unfocused
                 const uint32_t m = 0u - (uint32_t)(s_code[k] & 1u);
                 const uint32_t qq = s_col[k] & m;
                 count -= m;   // +1 when taken
-                b += qq & 0xFF;
-                g += (qq >> 8) & 0xFF;
-                r += (qq >> 16) & 0xFF;
+                if (PACKED) {
+                    b += qq & 0x00FF00FFu;   // blue in the low half, red in the high half
+                    g += (qq >> 8) & 0xFF;
+                } else {
+                    b += qq & 0xFF;
+                    g += (qq >> 8) & 0xFF;
+                    r += (qq >> 16) & 0xFF;
+                }
             };
             if (J.post == 1) {   // box [x-r, x+r) x [y-r, y+r)
                 for (int j = -radius; j < radius; j++)
@@ -1656,6 +1664,10 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restric
 #pragma unroll
                     for (int e = -1; e <= 1; e++)
                         tap(c + a * dr + e * radius);
+            }
+            if (PACKED) {
+                r = b >> 16;
+                b &= 0xFFFFu;
             }
             if (count)
                 out = (uint32_t)(uint8_t)div_count(b, count) | ((uint32_t)(uint8_t)div_count(g, count) << 8) |
