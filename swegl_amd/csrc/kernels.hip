@@ -388,10 +388,12 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B, const JobDev *__r
         }
         uint2 r = make_uint2(RECT_EMPTY, 0);
         if (have) {
-            tx0 = (E.x0 - J.V.vx) / J.tw;
-            tx1 = (E.x1 - J.V.vx) / J.tw;
-            ty0 = (E.y0 - J.V.vy) / J.th;
-            const int ty1 = (E.y1 - 1 - J.V.vy) / J.th;
+            // tile edges are powers of two and the offsets are not negative: shifts
+            const int sx = __builtin_ctz((unsigned)J.tw), sy = __builtin_ctz((unsigned)J.th);
+            tx0 = (E.x0 - J.V.vx) >> sx;
+            tx1 = (E.x1 - J.V.vx) >> sx;
+            ty0 = (E.y0 - J.V.vy) >> sy;
+            const int ty1 = (E.y1 - 1 - J.V.vy) >> sy;
             r = make_uint2((uint32_t)tx0 | ((uint32_t)ty0 << 16), (uint32_t)tx1 | ((uint32_t)ty1 << 16));
             ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
             if (ntiles > (int)B.big_tiles) {
@@ -414,7 +416,8 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B, const JobDev *__r
         uint32_t key = 0;
         if (act) {
             const int w = tx1 - tx0 + 1;
-            key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
+            const int ky = k == 0 ? 0 : k / w, kx = k == 0 ? 0 : k % w;   // k is the same in every lane
+            key = J.tile_base + (ty0 + ky) * J.tiles_x + (tx0 + kx);
             entries++;
         }
         wave_add<false>(B.tile_count, key * SW_TILE_PAD, act);
@@ -534,7 +537,8 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__re
         uint32_t key = 0;
         if (act) {
             const int w = tx1 - tx0 + 1;
-            key = J.tile_base + (ty0 + k / w) * J.tiles_x + (tx0 + k % w);
+            const int ky = k == 0 ? 0 : k / w, kx = k == 0 ? 0 : k % w;   // k is the same in every lane
+            key = J.tile_base + (ty0 + ky) * J.tiles_x + (tx0 + kx);
         }
         // the entry's place: the tile's offset (read while the rank's atomic is
         // in flight) plus its rank among the tile's entries so far
