@@ -13,7 +13,10 @@ from swegl_amd import _abi as A
 from swegl_amd.host import SceneDescHolder
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# SWEGL_ORACLE_LIB points the binding at another build of refcpu.cpp: the
+# planted-mistake test (tests/test_oracle_mutations.py) loads mutated copies
+# through it to show that the pin tests reject each mistake.
+LIB_PATH = os.environ.get("SWEGL_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 
@@ -49,6 +52,7 @@ def lib():
                                     C.POINTER(OracleStats)]
         L.oracle_transform.argtypes = [A.FP, A.FP, A.FP]
         L.oracle_vertex.argtypes = [A.FP] * 9
+        L.oracle_clip_vertex.argtypes = [A.FP, A.FP, C.c_float, A.FP, A.FP, A.FP, A.FP, A.FP]
         L.oracle_interp.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, A.FP]
         L.oracle_raster_triangle.restype = C.c_int
         L.oracle_raster_triangle.argtypes = [A.FP, C.c_int, C.c_int, A.U32P, A.FP]

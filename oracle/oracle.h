@@ -63,6 +63,13 @@ void oracle_transform(const float v[3], const float m[16], float out[3]);
 void oracle_vertex(const float v[3], const float n[3], const float M[16],
                    const float RS[16], const float view[16], const float proj[16],
                    float v_world[3], float n_world[3], float v_frustum[3]);
+/* make_clip_vertex (renderer.cpp:290-297) between a and b = {v_world xyz,
+ * normal xyz, tex uv} at `cut`, then world_to_viewport with node RS, view,
+ * projection and viewport matrix vpm = {m00, m03, m11, m13}.
+ * out = {v_world xyz, tex uv, normal_world xyz, v_viewport xyz}. */
+void oracle_clip_vertex(const float a[8], const float b[8], float cut, const float RS[16],
+                        const float view[16], const float proj[16], const float vpm[4],
+                        float out[11]);
 /* interpolator_g<1>: InitSelf(d,z1,z2); DisplaceStartingPoint(m); Step()
  * `steps` times.  out = {progress, value}. (interpolator.hpp:52-104) */
 void oracle_interp(float d, float z1, float z2, float m, int steps, float out[2]);

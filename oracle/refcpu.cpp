@@ -1470,6 +1470,38 @@ void oracle_vertex(const float v[3], const float n[3], const float M[16], const 
     v_frustum[2] = mv.v_viewport.z;
 }
 
+/* make_clip_vertex between a and b at `cut` (renderer.cpp:290-297): a, b =
+ * {v_world xyz, normal xyz, tex uv}; node rotation/scale RS, view and
+ * projection, viewport matrix {m00, m03, m11, m13}.  out = {v_world xyz,
+ * tex uv, normal_world xyz, v_viewport xyz}. */
+void oracle_clip_vertex(const float a[8], const float b[8], float cut, const float RS[16],
+                        const float view[16], const float proj[16], const float vpm[4],
+                        float out[11])
+{
+    node nd;
+    std::memcpy(nd.rs, RS, sizeof nd.rs);
+    viewport_m vp{};
+    vp.view = view;
+    vp.proj = proj;
+    vp.m00 = vpm[0];
+    vp.m03 = vpm[1];
+    vp.m11 = vpm[2];
+    vp.m13 = vpm[3];
+    std::vector<mesh_vertex> vs(2);
+    const float *src[2] = {a, b};
+    for (int i = 0; i < 2; i++) {
+        vs[i].v_world = {src[i][0], src[i][1], src[i][2]};
+        vs[i].normal = {src[i][3], src[i][4], src[i][5]};
+        vs[i].tex = {src[i][6], src[i][7]};
+    }
+    make_clip_vertex(vs, 0, 1, cut, nd, vp);
+    const mesh_vertex &nv = vs[2];
+    const float o[11] = {nv.v_world.x, nv.v_world.y, nv.v_world.z, nv.tex.x, nv.tex.y,
+                         nv.normal_world.x, nv.normal_world.y, nv.normal_world.z,
+                         nv.v_viewport.x, nv.v_viewport.y, nv.v_viewport.z};
+    std::memcpy(out, o, sizeof o);
+}
+
 void oracle_interp(float d, float z1, float z2, float m, int steps, float out[2])
 {
     interp it;

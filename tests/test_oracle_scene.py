@@ -178,3 +178,68 @@ def test_inverted_triangle_negates_normals():
     px = np.unique(fr.pixels[fr.tid != 0xFFFFFFFF])
     # light 0.5: texel (100,150,200) halves
     assert len(px) == 1 and px[0] == pixel_colors(50, 75, 100, 255)
+
+
+def test_frustum_side_met_by_any_vertex():
+    """inside_camera_frustum (renderer.cpp:58-70) asks, per side, that at
+    least one vertex be inside it, not that every vertex be: a triangle with
+    one vertex left of x' = -1 is drawn, and a triangle around the whole
+    view, no vertex of which lies inside, covers every pixel."""
+    # world z = 0 is at z' = (10/9.5)(3 - 0.5): x' = x / 2.63, so x = -4 is left of -1
+    fr = oracle.render(flat_scene([[[-4, -0.5, 0], B_, C_]]), [view()])[0]
+    assert fr.stats["drawn_subtriangles"] == 1
+    assert (fr.tid != 0xFFFFFFFF).sum() > 300
+    big = [[-20, -20, 0], [20, -20, 0], [0, 20, 0]]
+    fr = oracle.render(flat_scene([big]), [view()])[0]
+    assert fr.stats["drawn_subtriangles"] == 1
+    assert (fr.tid != 0xFFFFFFFF).all()
+
+
+def _identity_view(w, h):
+    """A viewport whose view and projection are identities: z' is the world z
+    exactly and x' = x / |z|, y' = y / |z|."""
+    vp = viewport_t(0, 0, w, h, Surface(w, h), pixel_shader_light_and_texture(), 0)
+    vp.m_camera.m_viewmatrix[:] = np.eye(4, dtype=np.float32)
+    vp.m_camera.m_projectionmatrix[:] = np.eye(4, dtype=np.float32)
+    vp.set_post_shader(post_shader_t())
+    return vp
+
+
+def test_near_rejection_compares_in_double():
+    """The near tests compare the float z' with the double literal 0.001
+    (renderer.cpp:489, :58-70, :281): 0.001f is above 0.001, so a triangle of
+    constant depth 0.001f is marked, set up and every fragment kept (a float
+    literal 0.001f would reject them all); at the float below, nothing is
+    marked."""
+    z = np.float32(0.001)
+    for depth, drawn in ((z, True), (np.nextafter(z, np.float32(0)), False)):
+        h = np.float32(0.5) * depth
+        tri = [[-h, -h, depth], [h, -h, depth], [0, h, depth]]
+        fr = oracle.render(flat_scene([tri], double_sided=True), [_identity_view(64, 64)])[0]
+        cov = fr.tid != 0xFFFFFFFF
+        if drawn:
+            assert fr.stats["fragments"] > 200 and cov.sum() == fr.stats["fragments"]
+            assert (fr.zbuffer[cov].view(np.uint32) == np.float32(depth).view(np.uint32)).all()
+        else:
+            assert fr.stats["drawn_subtriangles"] == 0 and not cov.any()
+
+
+def test_near_clip_cuts_at_the_near_plane():
+    """Only v0 in front (renderer.cpp:286-317): the edges are cut where
+    z' = 0.001, cut = (z0 - 0.001f) / (z0 - z1).  With identity view and
+    projection the vertex (0, 0, 1) and the two behind it at (0.001, 0, -1)
+    and (0, 0.001, -1) are cut at 0.4995, whose x' = 0.0004995 / 0.001 ~ 0.4995:
+    on a 400x400 viewport the clipped triangle is the right triangle with
+    corners (200, 200), (299.9, 200), (200, 100.1), about 4,990 pixels.  A cut
+    at z' = 0 would leave a sub-pixel sliver (no divide at z' = 0)."""
+    tri = [[0, 0, 1], [0.001, 0, -1], [0, 0.001, -1]]
+    fr = oracle.render(flat_scene([tri], double_sided=True), [_identity_view(400, 400)])[0]
+    assert fr.stats["clipped_triangles"] == 1 and fr.stats["drawn_subtriangles"] == 1
+    cov = fr.tid != 0xFFFFFFFF
+    assert 4800 < cov.sum() < 5200
+    ys, xs = np.nonzero(cov)
+    assert xs.min() == 200 and 298 <= xs.max() <= 300
+    assert 100 <= ys.min() <= 102 and ys.max() == 199
+    # depths run from 1 at the front vertex down to the plane
+    z = fr.zbuffer[cov]
+    assert z.max() <= 1 and z.min() < np.float32(0.0011) and (z > np.float32(0.001)).all()
