@@ -12,6 +12,7 @@
 #include <map>
 #include <unordered_map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -119,6 +120,40 @@ struct StageSlot {
 // into the graph's kernel arguments.
 struct Resources {
     DevBuf jobs, tile_count, tile_offset, bins, ctr;
+    size_t bytes() const { return jobs.n + tile_count.n + tile_offset.n + bins.n + ctr.n; }
+};
+
+// The addresses a launch sequence runs on: a Resources set, or regions of
+// the allocation-free entry point's fixed scratch array.
+struct ResPtr {
+    char *jobs = nullptr;
+    uint32_t *tile_count = nullptr, *tile_offset = nullptr, *bins = nullptr, *ctr = nullptr;
+};
+static ResPtr res_ptr(Resources &R)
+{
+    ResPtr p;
+    p.jobs = R.jobs.as<char>();
+    p.tile_count = R.tile_count.as<uint32_t>();
+    p.tile_offset = R.tile_offset.as<uint32_t>();
+    p.bins = R.bins.as<uint32_t>();
+    p.ctr = R.ctr.as<uint32_t>();
+    return p;
+}
+
+// Bump allocation over a fixed device region (the allocation-free path):
+// take() hands out 256-byte aligned pieces and fails once the region is full.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0, want = 0;   // want: bytes asked for, met or not
+    void *take(size_t bytes)
+    {
+        const size_t off = (used + 255) & ~(size_t)255;
+        want = off + bytes;
+        if (!base || off + bytes > cap)
+            return nullptr;
+        used = off + bytes;
+        return base + off;
+    }
 };
 
 // A captured launch sequence.  Only the resource addresses and grid sizes
@@ -134,6 +169,8 @@ struct GraphEntry {
     int next = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;   // swegl_ctx::graph_tick of the last replay (LRU)
+    size_t bytes() const { return res.bytes() + 2 * job_bytes; }
     ~GraphEntry()
     {
         if (exec)
@@ -181,7 +218,15 @@ struct swegl_ctx {
     std::vector<hipEvent_t> event_pool;
     Lane lane0;   // the context stream
     std::map<std::string, std::unique_ptr<GraphEntry>> graphs;
+    uint64_t graph_tick = 0;
+    size_t graph_bytes = 0;                // device + pinned bytes of the cached graphs
+    std::vector<Lane *> server_lanes;      // lanes of this context's servers (scene scratch)
 };
+
+// The graph cache keeps at most this many captured launch sequences and
+// this many bytes of their resources; the least recently replayed go first.
+static const size_t kGraphCacheEntries = 64;
+static const size_t kGraphCacheBytes = (size_t)4 << 30;
 
 // --------------------------------------------------------------- helpers
 static hipEvent_t take_event(swegl_ctx *c)
@@ -297,6 +342,27 @@ Lane::~Lane()
         (void)hipStreamDestroy(stream);
 }
 
+// Once per device, with the device current: every kernel resolved (code
+// object loaded, symbols looked up) before any stream capture, so that no
+// kernel's first use happens inside a capture.
+static const int kMaxDevices = 64;
+static std::mutex g_device_mu;
+static bool g_kernels_resolved[kMaxDevices];
+
+static int device_setup(int device)
+{
+    if (device >= kMaxDevices)
+        return fail(SWEGL_ENODEV, "device %d above the library's limit of %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> g(g_device_mu);
+    if (!g_kernels_resolved[device]) {
+        hipError_t e = resolve_kernels();
+        if (e != hipSuccess)
+            return fail(SWEGL_EIO, "resolving the kernels on device %d: %s", device, hipGetErrorString(e));
+        g_kernels_resolved[device] = true;
+    }
+    return SWEGL_OK;
+}
+
 // ------------------------------------------------------------------ ctx
 extern "C" const char *swegl_last_error(void) { return g_err.c_str(); }
 
@@ -316,6 +382,9 @@ extern "C" int swegl_ctx_create(int device, swegl_ctx **out)
     std::unique_ptr<swegl_ctx> c(new swegl_ctx);
     c->device = device;
     c->n_cus = prop.multiProcessorCount;
+    int src = device_setup(device);
+    if (src)
+        return src;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc((void **)&c->h_stats, SW_CTR_CONTROL * sizeof(uint32_t), hipHostMallocDefault));
     c->lane0.stream = c->stream;
@@ -341,6 +410,7 @@ extern "C" int swegl_ctx_destroy(swegl_ctx *c)
         (void)hipHostFree(c->h_stage);
     c->graphs.clear();
     (void)hipStreamDestroy(c->stream);
+    c->lane0.stream = nullptr;   // destroyed above: the lane must not wait on it
     delete c;
     return SWEGL_OK;
 }
@@ -930,7 +1000,61 @@ static void wave_tile(const swegl_ctx *c, const std::vector<JobIn> &jobs, int &t
     }
 }
 
-static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wave &W)
+// Per-viewport scratch of one job: frustum coordinates, marks, tile
+// rectangles, the big list, and with DoF the pre-DoF colour and code bytes,
+// with layers the flattened layer colour.
+struct JobScratch {
+    float4 *ndc = nullptr;
+    uint8_t *yes = nullptr;
+    uint2 *rect = nullptr;
+    uint32_t *big = nullptr;
+    uint32_t *dof = nullptr;
+    uint32_t *tflat = nullptr;
+};
+static size_t job_scratch_sizes(const swegl_scene *s, const swegl_viewport_desc &v, size_t sz[6])
+{
+    const size_t V = std::max<uint32_t>(1, s->n_vertices), T = std::max<uint32_t>(1, s->n_tris);
+    const size_t P = (size_t)v.w * v.h;
+    sz[0] = V * 16;
+    sz[1] = V;
+    sz[2] = T * 8;
+    sz[3] = T * 4;
+    sz[4] = v.post_shader != SWEGL_POST_NULL ? P * 5 : 0;   // colour, then one code byte per pixel
+    sz[5] = v.transparency_layer_count > 0 ? P * 4 : 0;
+    return sz[0] + sz[1] + sz[2] + sz[3] + sz[4] + sz[5];
+}
+static int job_scratch(Lane &L, Arena *arena, swegl_scene *s, int use, const swegl_viewport_desc &v,
+                       JobScratch &out)
+{
+    size_t sz[6];
+    job_scratch_sizes(s, v, sz);
+    void *p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (arena) {
+        for (int k = 0; k < 6; k++)
+            if (sz[k] && !(p[k] = arena->take(sz[k])))
+                return fail(SWEGL_ERANGE, "allocation-free render: the viewports' scratch needs more than "
+                                          "the %zu bytes of the fixed scratch array", arena->cap);
+    } else {
+        ViewScratch &vs = *L.scratch(s, use);
+        DevBuf *b[6] = {&vs.ndc, &vs.yes, &vs.rect, &vs.big, &vs.dof, &vs.tflat};
+        for (int k = 0; k < 6; k++)
+            if (sz[k]) {
+                int rc = b[k]->ensure(sz[k]);
+                if (rc)
+                    return rc;
+                p[k] = b[k]->p;
+            }
+    }
+    out.ndc = (float4 *)p[0];
+    out.yes = (uint8_t *)p[1];
+    out.rect = (uint2 *)p[2];
+    out.big = (uint32_t *)p[3];
+    out.dof = (uint32_t *)p[4];
+    out.tflat = (uint32_t *)p[5];
+    return SWEGL_OK;
+}
+
+static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wave &W, Arena *arena)
 {
     const uint32_t nj = (uint32_t)jobs.size();
     W.jd.assign(nj, JobDev{});
@@ -940,17 +1064,11 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
     for (uint32_t j = 0; j < nj; j++) {
         swegl_scene *s = jobs[j].scene;
         const swegl_viewport_desc &v = *jobs[j].vp;
-        ViewScratch &vs = *L.scratch(s, scene_uses[s]++);
-        const size_t V = std::max<uint32_t>(1, s->n_vertices), T = std::max<uint32_t>(1, s->n_tris);
-        int rc;
-        if ((rc = vs.ndc.ensure(V * 16)) || (rc = vs.yes.ensure(V)) || (rc = vs.rect.ensure(T * 8)) ||
-            (rc = vs.big.ensure(T * 4)))
+        JobScratch vs;
+        int rc = job_scratch(L, arena, s, scene_uses[s]++, v, vs);
+        if (rc)
             return rc;
         const bool dof = v.post_shader != SWEGL_POST_NULL;
-        if (v.transparency_layer_count > 0 && (rc = vs.tflat.ensure((size_t)v.w * v.h * 4)))
-            return rc;
-        if (dof && (rc = vs.dof.ensure((size_t)v.w * v.h * 5)))   // colour, then one code byte per pixel
-            return rc;
         JobDev &J = W.jd[j];
         memset(&J, 0, sizeof J);
         J.px = s->px.as<float>();
@@ -976,10 +1094,10 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         J.sun_I = s->sun_I;
         J.world = s->world.as<float4>();
         J.nworld = s->nworld.as<float4>();
-        J.ndc = vs.ndc.as<float4>();
-        J.yes = vs.yes.as<uint8_t>();
-        J.rect = vs.rect.as<uint2>();
-        J.big = vs.big.as<uint32_t>();
+        J.ndc = vs.ndc;
+        J.yes = vs.yes;
+        J.rect = vs.rect;
+        J.big = vs.big;
         m34_from(v.view, J.V.view);
         m34_from(v.projection, J.V.proj);
         hm::viewport_matrix(v.x, v.y, v.w, v.h, &J.V.m00, &J.V.m03, &J.V.m11, &J.V.m13);
@@ -992,8 +1110,8 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         J.surface_pitch = v.color->w;
         J.dofc = nullptr;
         if (dof) {
-            J.out_color = vs.dof.as<uint32_t>();
-            J.dofc = (uint8_t *)vs.dof.p + (size_t)v.w * v.h * 4;
+            J.out_color = vs.dof;
+            J.dofc = (uint8_t *)vs.dof + (size_t)v.w * v.h * 4;
             J.out_pitch = v.w;
             J.out_ox = v.x;
             J.out_oy = v.y;
@@ -1007,7 +1125,7 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         J.tid = v.triangle_id ? v.triangle_id->buf.as<uint32_t>() : nullptr;
         J.shader = v.pixel_shader;
         J.layers = v.transparency_layer_count;
-        J.tflat = v.transparency_layer_count > 0 ? vs.tflat.as<uint32_t>() : nullptr;
+        J.tflat = v.transparency_layer_count > 0 ? vs.tflat : nullptr;
         W.layers = W.layers || v.transparency_layer_count > 0;
         W.generic = W.generic || v.pixel_shader != SWEGL_SHADER_PHONG_BILINEAR;
         J.post = v.post_shader;
@@ -1054,39 +1172,55 @@ static size_t place_job_tables(std::vector<Wave> &waves)
     return job_bytes;
 }
 
-// Size a resource set for a list of waves (grow-only).
-static int size_resources(Resources &R, std::vector<Wave> &waves)
+// Bytes of each launch resource for a list of waves: job tables, tile
+// counters (one padded line per tile), tile offsets, bins, counter words.
+// The counter block always holds the statistics and control words that
+// read_stats copies.
+static void resource_sizes(std::vector<Wave> &waves, size_t sz[5])
 {
     const size_t job_bytes = place_job_tables(waves);
-    uint32_t tiles = 0, n_ctr = 0;
+    uint32_t tiles = 0, n_ctr = SW_CTR_BIG0;
     uint64_t cap = 0;
     for (Wave &W : waves) {
         tiles = std::max(tiles, W.tiles);
         n_ctr = std::max(n_ctr, W.n_ctr);
         cap = std::max(cap, W.cap);
     }
+    sz[0] = job_bytes;
+    sz[1] = (size_t)tiles * SW_TILE_PAD * 4 + 8;
+    sz[2] = (size_t)tiles * 4 + 8;
+    sz[3] = cap * 4;
+    sz[4] = (size_t)n_ctr * 4;
+}
+
+// Size a resource set for a list of waves (grow-only).
+static int size_resources(Resources &R, std::vector<Wave> &waves)
+{
+    size_t sz[5];
+    resource_sizes(waves, sz);
     int rc;
-    if ((rc = R.jobs.ensure(job_bytes)) || (rc = R.tile_count.ensure((size_t)tiles * SW_TILE_PAD * 4 + 8)) ||
-        (rc = R.tile_offset.ensure((size_t)tiles * 4 + 8)) ||
-        (rc = R.bins.ensure(cap * 4)) || (rc = R.ctr.ensure((size_t)n_ctr * 4)))
+    if ((rc = R.jobs.ensure(sz[0])) || (rc = R.tile_count.ensure(sz[1])) || (rc = R.tile_offset.ensure(sz[2])) ||
+        (rc = R.bins.ensure(sz[3])) || (rc = R.ctr.ensure(sz[4])))
         return rc;
     return SWEGL_OK;
 }
 
 // Enqueue the kernels of one wave (no allocation: safe under graph capture).
-static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool first_wave, bool capturing,
+// `untimed`: no profiling events (capture, and the allocation-free path).
+static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bool first_wave, bool untimed,
                        bool want_stats)
 {
+    const bool capturing = untimed;
     BatchDev B;
-    B.jobs = (const JobDev *)((char *)R.jobs.p + W.job_offset);
+    B.jobs = (const JobDev *)(R.jobs + W.job_offset);
     B.n_jobs = (uint32_t)W.jd.size();
     B.n_tiles = W.tiles;
-    B.tile_count = R.tile_count.as<uint32_t>();
-    B.tile_offset = R.tile_offset.as<uint32_t>();
-    B.tile_cursor = R.tile_count.as<uint32_t>() + 1;
-    B.bins = R.bins.as<uint32_t>();
+    B.tile_count = R.tile_count;
+    B.tile_offset = R.tile_offset;
+    B.tile_cursor = R.tile_count + 1;
+    B.bins = R.bins;
     B.bin_capacity = (uint32_t)W.cap;
-    B.ctr = R.ctr.as<uint32_t>();
+    B.ctr = R.ctr;
     B.big_tiles = W.big_tiles;
     B.stats = want_stats ? 1u : 0u;
     // statistics are zeroed once per render by the first wave; control words
@@ -1116,7 +1250,8 @@ static int launch_wave(swegl_ctx *c, Lane &L, Resources &R, const Wave &W, bool 
     return SWEGL_OK;
 }
 
-static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::vector<Wave> &waves)
+static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::vector<Wave> &waves,
+                   Arena *arena = nullptr)
 {
     HIPCHK(hipSetDevice(c->device));
     for (const JobIn &j : all) {
@@ -1151,16 +1286,16 @@ static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::ve
         if (groups[w].empty())
             continue;
         waves.emplace_back();
-        int rc = build_wave(c, L, groups[w], waves.back());
+        int rc = build_wave(c, L, groups[w], waves.back(), arena);
         if (rc)
             return rc;
     }
     return SWEGL_OK;
 }
 
-static int read_stats(swegl_ctx *c, Lane &L, Resources &R, swegl_frame_stats *stats)
+static int read_stats(swegl_ctx *c, Lane &L, const uint32_t *ctr, swegl_frame_stats *stats)
 {
-    HIPCHK(hipMemcpyAsync(c->h_stats, R.ctr.p, SW_CTR_CONTROL * sizeof(uint32_t), hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(c->h_stats, ctr, SW_CTR_CONTROL * sizeof(uint32_t), hipMemcpyDeviceToHost,
                           L.stream));
     HIPCHK(hipStreamSynchronize(L.stream));
     drain_profile(c);
@@ -1178,22 +1313,34 @@ static int read_stats(swegl_ctx *c, Lane &L, Resources &R, swegl_frame_stats *st
     return SWEGL_OK;
 }
 
+// No viewport, no wave: the statistics are zero (nothing ran to count them).
+static int no_waves(swegl_frame_stats *stats)
+{
+    if (stats)
+        memset(stats, 0, sizeof *stats);
+    return SWEGL_OK;
+}
+
 static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swegl_frame_stats *stats)
 {
     std::vector<Wave> waves;
     int rc = prepare(c, L, all, waves);
     if (rc)
         return rc;
+    if (waves.empty())
+        return no_waves(stats);
     if ((rc = size_resources(L.res, waves)))
         return rc;
+    const ResPtr R = res_ptr(L.res);
     for (size_t w = 0; w < waves.size(); w++) {
-        if ((rc = upload(L, (char *)L.res.jobs.p + waves[w].job_offset, waves[w].jd.data(),
+        if ((rc = upload(L, R.jobs + waves[w].job_offset, waves[w].jd.data(),
                          waves[w].jd.size() * sizeof(JobDev))) ||
-            (rc = launch_wave(c, L, L.res, waves[w], w == 0, false, stats != nullptr)))
+            (rc = launch_wave(c, L, R, waves[w], w == 0, false, stats != nullptr)))
             return rc;
     }
-    return stats ? read_stats(c, L, L.res, stats) : SWEGL_OK;
+    return stats ? read_stats(c, L, R.ctr, stats) : SWEGL_OK;
 }
+
 
 // ----------------------------------------------------------- graph path
 
@@ -1216,16 +1363,40 @@ static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
     return k;
 }
 
+// Before a new capture: drop the least recently replayed graphs while the
+// cache is at its entry or byte limit.  A dropped graph may still be running
+// or be about to read its pinned tables, so the stream is drained first.
+static int evict_graphs(swegl_ctx *c)
+{
+    if (c->graphs.size() < kGraphCacheEntries && c->graph_bytes < kGraphCacheBytes)
+        return SWEGL_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    while (!c->graphs.empty() &&
+           (c->graphs.size() >= kGraphCacheEntries || c->graph_bytes >= kGraphCacheBytes)) {
+        auto oldest = c->graphs.begin();
+        for (auto it = c->graphs.begin(); it != c->graphs.end(); ++it)
+            if (it->second->last_use < oldest->second->last_use)
+                oldest = it;
+        c->graph_bytes -= std::min(c->graph_bytes, oldest->second->bytes());
+        c->graphs.erase(oldest);
+    }
+    return SWEGL_OK;
+}
+
 static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
 {
     std::vector<Wave> waves;
     int rc = prepare(c, c->lane0, all, waves);
     if (rc)
         return rc;
+    if (waves.empty())
+        return no_waves(stats);
     const std::string key = graph_key(c, waves) + (stats ? "/stats" : "");
     auto it = c->graphs.find(key);
     GraphEntry *G;
     if (it == c->graphs.end()) {
+        if ((rc = evict_graphs(c)))
+            return rc;
         std::unique_ptr<GraphEntry> g(new GraphEntry);
         if ((rc = size_resources(g->res, waves)))
             return rc;
@@ -1238,17 +1409,19 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int lrc = SWEGL_OK;
+        const ResPtr GR = res_ptr(g->res);
         for (size_t w = 0; lrc == SWEGL_OK && w < waves.size(); w++)
-            lrc = launch_wave(c, c->lane0, g->res, waves[w], w == 0, true, stats != nullptr);
+            lrc = launch_wave(c, c->lane0, GR, waves[w], w == 0, true, stats != nullptr);
         hipGraph_t graph = nullptr;
         hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        g->graph = graph;   // destroyed with the entry if anything below fails
         if (lrc)
             return lrc;
         if (ee != hipSuccess)
             return fail(SWEGL_EIO, "hipStreamEndCapture: %s", hipGetErrorString(ee));
-        g->graph = graph;
         HIPCHK(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
         G = g.get();
+        c->graph_bytes += G->bytes();
         c->graphs[key] = std::move(g);
     } else {
         G = it->second.get();
@@ -1272,7 +1445,8 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
     HIPCHK(hipGraphLaunch(G->exec, c->stream));
     HIPCHK(hipEventRecord(G->done[slot], c->stream));
     G->used[slot] = true;
-    return stats ? read_stats(c, c->lane0, G->res, stats) : SWEGL_OK;
+    G->last_use = ++c->graph_tick;
+    return stats ? read_stats(c, c->lane0, G->res.ctr.as<uint32_t>(), stats) : SWEGL_OK;
 }
 
 extern "C" int swegl_render(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vps, uint32_t n,

@@ -1785,6 +1785,30 @@ hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStr
     return hipSuccess;
 }
 
+hipError_t resolve_kernels()
+{
+    const void *k[] = {
+        (const void *)k_world, (const void *)k_vertex, (const void *)k_mark, (const void *)k_bin_count,
+        (const void *)k_scan, (const void *)k_bin_fill,
+        (const void *)k_raster<8, 8, false>, (const void *)k_raster<16, 16, false>,
+        (const void *)k_raster<32, 16, false>, (const void *)k_raster<32, 32, false>,
+        (const void *)k_raster<64, 32, false>, (const void *)k_raster<64, 64, false>,
+        (const void *)k_raster<8, 8, true>, (const void *)k_raster<16, 16, true>,
+        (const void *)k_raster<32, 16, true>, (const void *)k_raster<32, 32, true>,
+        (const void *)k_raster<64, 32, true>, (const void *)k_raster<64, 64, true>,
+        (const void *)k_raster_layers<false>, (const void *)k_raster_layers<true>,
+        (const void *)k_tflatten, (const void *)k_tcopy,
+        (const void *)k_dof<8>, (const void *)k_dof<SW_MAX_DOF_RADIUS>, (const void *)k_fill32,
+    };
+    for (const void *f : k) {
+        hipFuncAttributes a;
+        hipError_t e = hipFuncGetAttributes(&a, f);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_fill32(uint32_t *p, uint64_t n, uint32_t word, hipStream_t s)
 {
     uint64_t blocks = (n + 255) / 256;

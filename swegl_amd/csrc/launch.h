@@ -15,4 +15,8 @@ hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups,
 // layer raster, then the flatten over the screen and the copy back.
 hipError_t launch_layers(const BatchDev &B, bool generic, uint32_t pixels, hipStream_t s);
 hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStream_t s);
+// Resolve every kernel of this file on the current device (code object
+// loaded, each kernel's symbol looked up) without launching any, so that no
+// kernel's first use happens inside a stream capture.
+hipError_t resolve_kernels();
 }  // namespace sw
