@@ -263,6 +263,25 @@ int  swegl_render(swegl_ctx *ctx, swegl_scene *scene,
 int  swegl_debug_vertex_state(swegl_ctx *ctx, swegl_scene *scene, const swegl_viewport_desc *viewport,
                               float *v_world, float *normal_world, float *v_viewport, uint8_t *yes);
 
+/* Allocation-free variant of swegl_render.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels run on a fixed per-device scratch array of
+ * SWEGL_NOALLOC_SCRATCH_BYTES (held by every device the library runs on
+ * until the process ends) instead of buffers the library allocates; the
+ * scene and the viewports' targets are the caller's, as in swegl_render.
+ * The call allocates no device or pinned memory.  Renders whose scratch
+ * does not fit (per viewport: 17 B per vertex, 12 B per triangle, 5 B per
+ * pixel with DoF, 4 B per pixel with layers; per call: the bins, 4 B per
+ * triangle and bin-limit tile, and the tile counters) return SWEGL_ERANGE
+ * before anything is enqueued.  Every call on a device shares its array:
+ * calls from contexts that overlap on one device race; calls on one context
+ * are ordered by its stream. */
+#define SWEGL_NOALLOC_SCRATCH_BYTES ((uint64_t)64 << 20)
+int  swegl_render_noalloc(swegl_ctx *ctx, swegl_scene *scene,
+                          const swegl_viewport_desc *viewports, uint32_t n_viewports,
+                          swegl_frame_stats *stats);
+
 /* Plain path as one replayed hipGraph: the first call for a given
  * (scene, viewport set) captures, later calls replay. */
 int  swegl_render_graph(swegl_ctx *ctx, swegl_scene *scene,

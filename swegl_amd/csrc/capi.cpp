@@ -221,6 +221,10 @@ struct swegl_ctx {
     uint64_t graph_tick = 0;
     size_t graph_bytes = 0;                // device + pinned bytes of the cached graphs
     std::vector<Lane *> server_lanes;      // lanes of this context's servers (scene scratch)
+    char *scratch = nullptr;               // this device's fixed scratch array (allocation-free path)
+    std::vector<char> noalloc_tables;      // host job tables of the allocation-free path
+    hipEvent_t noalloc_done = nullptr;     // their last upload
+    bool noalloc_used = false;
 };
 
 // The graph cache keeps at most this many captured launch sequences and
@@ -342,24 +346,42 @@ Lane::~Lane()
         (void)hipStreamDestroy(stream);
 }
 
-// Once per device, with the device current: every kernel resolved (code
-// object loaded, symbols looked up) before any stream capture, so that no
-// kernel's first use happens inside a capture.
+// ------------------------------------------------- allocation-free scratch
+// swegl_render_noalloc runs on this array instead of buffers the library
+// allocates: SWEGL_NOALLOC_SCRATCH_BYTES on every device the library runs
+// on, held from the code object's load until the process ends.  In host
+// code the array's name is not a device address: swegl_ctx_create looks up
+// each device's address once (hipGetSymbolAddress, with that device current)
+// and every use goes through that address.
+static __device__ __align__(256) char scratch[SWEGL_NOALLOC_SCRATCH_BYTES];
 static const int kMaxDevices = 64;
 static std::mutex g_device_mu;
+static char *g_scratch_ptr[kMaxDevices];   // per device: the address of `scratch`
 static bool g_kernels_resolved[kMaxDevices];
 
-static int device_setup(int device)
+// Once per device, with the device current: the scratch array's address,
+// and every kernel resolved (code object loaded, symbols looked up) before
+// any stream capture, so that no kernel's first use falls inside a capture.
+static int device_setup(int device, char **scratch_out)
 {
     if (device >= kMaxDevices)
         return fail(SWEGL_ENODEV, "device %d above the library's limit of %d", device, kMaxDevices);
     std::lock_guard<std::mutex> g(g_device_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SWEGL_EIO, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                        hipGetErrorString(e));
+        g_scratch_ptr[device] = (char *)p;
+    }
     if (!g_kernels_resolved[device]) {
         hipError_t e = resolve_kernels();
         if (e != hipSuccess)
             return fail(SWEGL_EIO, "resolving the kernels on device %d: %s", device, hipGetErrorString(e));
         g_kernels_resolved[device] = true;
     }
+    *scratch_out = g_scratch_ptr[device];
     return SWEGL_OK;
 }
 
@@ -382,11 +404,12 @@ extern "C" int swegl_ctx_create(int device, swegl_ctx **out)
     std::unique_ptr<swegl_ctx> c(new swegl_ctx);
     c->device = device;
     c->n_cus = prop.multiProcessorCount;
-    int src = device_setup(device);
+    int src = device_setup(device, &c->scratch);
     if (src)
         return src;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc((void **)&c->h_stats, SW_CTR_CONTROL * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&c->noalloc_done, hipEventDisableTiming));
     c->lane0.stream = c->stream;
     int lrc = c->lane0.init(false);
     if (lrc)
@@ -404,6 +427,8 @@ extern "C" int swegl_ctx_destroy(swegl_ctx *c)
     drain_profile(c);
     for (hipEvent_t e : c->event_pool)
         (void)hipEventDestroy(e);
+    if (c->noalloc_done)
+        (void)hipEventDestroy(c->noalloc_done);
     if (c->h_stats)
         (void)hipHostFree(c->h_stats);
     if (c->h_stage)
@@ -1205,6 +1230,24 @@ static int size_resources(Resources &R, std::vector<Wave> &waves)
     return SWEGL_OK;
 }
 
+// The same resources carved from an arena (the allocation-free path).
+static int carve_resources(Arena &A, std::vector<Wave> &waves, ResPtr &out)
+{
+    size_t sz[5];
+    resource_sizes(waves, sz);
+    void *p[5];
+    for (int k = 0; k < 5; k++)
+        if (!(p[k] = A.take(sz[k])))
+            return fail(SWEGL_ERANGE, "allocation-free render needs %zu bytes of scratch or more, the fixed "
+                                      "scratch array holds %zu", A.want, A.cap);
+    out.jobs = (char *)p[0];
+    out.tile_count = (uint32_t *)p[1];
+    out.tile_offset = (uint32_t *)p[2];
+    out.bins = (uint32_t *)p[3];
+    out.ctr = (uint32_t *)p[4];
+    return SWEGL_OK;
+}
+
 // Enqueue the kernels of one wave (no allocation: safe under graph capture).
 // `untimed`: no profiling events (capture, and the allocation-free path).
 static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bool first_wave, bool untimed,
@@ -1341,6 +1384,47 @@ static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swe
     return stats ? read_stats(c, L, R.ctr, stats) : SWEGL_OK;
 }
 
+// The allocation-free variant of render_jobs: every piece of device scratch
+// comes from the fixed array (per-viewport scratch first, then the launch
+// resources), the job tables go in by a copy from host memory the call owns,
+// and nothing is allocated on the device or pinned on the host.  Sizes that
+// do not fit are refused before anything is enqueued.
+static int render_jobs_noalloc(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
+{
+    if (!c->scratch)
+        return fail(SWEGL_EIO, "no scratch array address for device %d", c->device);
+    Arena A;
+    A.base = c->scratch;
+    A.cap = SWEGL_NOALLOC_SCRATCH_BYTES;
+    std::vector<Wave> waves;
+    int rc = prepare(c, c->lane0, all, waves, &A);
+    if (rc)
+        return rc;
+    if (waves.empty())
+        return no_waves(stats);
+    ResPtr R;
+    if ((rc = carve_resources(A, waves, R)))
+        return rc;
+    size_t job_bytes = 0;
+    for (const Wave &W : waves)
+        job_bytes = std::max(job_bytes, W.job_offset + W.jd.size() * sizeof(JobDev));
+    // the host copy of the tables is rewritten only once the previous call's
+    // upload from it has completed
+    if (c->noalloc_used)
+        HIPCHK(hipEventSynchronize(c->noalloc_done));
+    std::vector<char> &tables = c->noalloc_tables;
+    if (tables.size() < job_bytes)
+        tables.resize(job_bytes);
+    for (const Wave &W : waves)
+        memcpy(tables.data() + W.job_offset, W.jd.data(), W.jd.size() * sizeof(JobDev));
+    HIPCHK(hipMemcpyAsync(R.jobs, tables.data(), job_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->noalloc_done, c->stream));
+    c->noalloc_used = true;
+    for (size_t w = 0; w < waves.size(); w++)
+        if ((rc = launch_wave(c, c->lane0, R, waves[w], w == 0, true, stats != nullptr)))
+            return rc;
+    return stats ? read_stats(c, c->lane0, R.ctr, stats) : SWEGL_OK;
+}
 
 // ----------------------------------------------------------- graph path
 
@@ -1458,6 +1542,19 @@ extern "C" int swegl_render(swegl_ctx *c, swegl_scene *s, const swegl_viewport_d
     for (uint32_t i = 0; i < n; i++)
         jobs.push_back({s, &vps[i]});
     return render_jobs(c, c->lane0, jobs, stats);
+}
+
+// Allocation-free variant of swegl_render (no counterpart in the reference):
+// the same kernels on the device's fixed scratch array.
+extern "C" int swegl_render_noalloc(swegl_ctx *c, swegl_scene *s, const swegl_viewport_desc *vps, uint32_t n,
+                                    swegl_frame_stats *stats)
+{
+    if (!c || !s || (n && !vps))
+        return fail(SWEGL_EINVAL, "null argument");
+    std::vector<JobIn> jobs;
+    for (uint32_t i = 0; i < n; i++)
+        jobs.push_back({s, &vps[i]});
+    return render_jobs_noalloc(c, jobs, stats);
 }
 
 // The scene fields the reference's render writes (model.hpp:20-29), for

@@ -58,6 +58,8 @@ def lib():
                                        C.POINTER(A.FrameStats)]),
             "swegl_render_graph": (C.c_int, [vp, vp, C.POINTER(A.ViewportDesc), C.c_uint32,
                                              C.POINTER(A.FrameStats)]),
+            "swegl_render_noalloc": (C.c_int, [vp, vp, C.POINTER(A.ViewportDesc), C.c_uint32,
+                                               C.POINTER(A.FrameStats)]),
             "swegl_render_batch": (C.c_int, [vp, C.POINTER(A.Job), C.c_uint32,
                                              C.POINTER(A.FrameStats)]),
             "swegl_render_batch_graph": (C.c_int, [vp, C.POINTER(A.Job), C.c_uint32,

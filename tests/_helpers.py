@@ -14,8 +14,9 @@ class GpuFrame:
         self.pixels, self.zbuffer, self.tid, self.stats = pixels, zbuffer, tid, stats
 
 
-def gpu_render(ctx, scene, viewports, dscene=None, with_tid=True):
-    """Render on the GPU; each viewport's surface starts from its host pixels."""
+def gpu_render(ctx, scene, viewports, dscene=None, with_tid=True, entry="swegl_render"):
+    """Render on the GPU through the C entry point `entry` (swegl_render or
+    swegl_render_noalloc); each viewport's surface starts from its host pixels."""
     holder = SceneDescHolder(scene)
     own = dscene is None
     if own:
@@ -34,7 +35,7 @@ def gpu_render(ctx, scene, viewports, dscene=None, with_tid=True):
         planes.append((surfaces[id(s)], z, tid))
         descs[i] = vp.desc(surfaces[id(s)], z, tid)
     st = (A.FrameStats * 1)()
-    check(lib().swegl_render(ctx.h, dscene.h, descs, len(viewports), st), "swegl_render")
+    check(getattr(lib(), entry)(ctx.h, dscene.h, descs, len(viewports), st), entry)
     frames = []
     stats = {k: getattr(st[0], k) for k, _ in A.FrameStats._fields_}
     for (t, z, tid) in planes:
