@@ -41,6 +41,33 @@ def algorithmic_bytes(scene, vp, f_vis):
     return frame, raster
 
 
+def job_bytes(scene, vps, f_vis):
+    """Algorithmic bytes of one job (a scene and its viewports): the scene's
+    vertex inputs and indices once, texels bounded by 16 B per shaded pixel
+    over all its viewports (f_vis: shaded pixels per viewport), colour and
+    depth written per viewport pixel and the DoF pass where it is on."""
+    V = scene.n_vertices()
+    I = sum(len(p.indices) for n in scene.nodes for p in n.primitives)
+    tex = sum(t.width * t.height * 4 for t in scene.images)
+    b = V * 32 + I * 4 + min(tex, 16 * int(sum(f_vis)))
+    for vp in vps:
+        P = vp.m_w * vp.m_h
+        dof = vp.m_post_shader is not None and vp.m_post_shader.kind != 0
+        b += P * 8 + (P * 12 if dof else 0)
+    return b
+
+
+def roofline_of(nbytes, seconds):
+    gbs = nbytes / seconds / 1e9
+    return {"bound": "hbm", "algorithmic_bytes": int(nbytes), "achieved": gbs, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+
+
+def shaded_pixels(z_target):
+    import numpy as np
+    return int((z_target.read().view(np.uint32) != 0x7F7F7F7F).sum())
+
+
 def _cpu_model():
     cpu = platform.processor() or platform.machine()
     try:
@@ -83,7 +110,13 @@ def cpu_baseline(scene, vps, budget_s, threads):
         nt = sum(f.result() for f in [ex.submit(worker) for _ in range(threads)])
     elt = time.perf_counter() - t0
     cpu = _cpu_model()
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
     return {"value": nt / elt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(), "usable_cpus": usable,
+            "threads_note": "one frame per thread on the host threads allotted to one GPU",
             "sample": f"{nt} frames of the same workload, one frame per thread on {threads} threads, "
                       f"{elt:.1f} s ({cpu})",
             "single_thread": {"frames_per_s": n1 / el1, "ms_per_frame": el1 * 1e3 / n1,
@@ -121,10 +154,34 @@ def fp64_reshade(ctx, A, check, L, ds, scene, vp):
         vp.set_post_shader(keep)
 
 
-def plain_vs_batched(ctx, A, check, L, pairs, iters):
+def fp64_reshade_jobs(ctx, A, check, L, jobs, label):
+    """fp64_reshade over every viewport of a list of (scene, viewports)
+    jobs, each viewport rendered alone: the largest and the pixel-weighted
+    mean colour error against the FP64 re-shade of the same winners."""
+    from swegl_amd.device import DeviceScene
+    from swegl_amd.host import SceneDescHolder
+
+    mx, tot, px, exact, same = 0, 0.0, 0, 0.0, True
+    for s, vps in jobs:
+        ds = DeviceScene(ctx, SceneDescHolder(s))
+        for vp in vps:
+            e = fp64_reshade(ctx, A, check, L, ds, s, vp)
+            mx = max(mx, e["max"])
+            tot += e["mean"] * e["pixels"]
+            exact += e.get("frac_exact", 1.0) * e["pixels"]
+            px += e["pixels"]
+            same = same and e["depth_and_ids_bit_exact"]
+        ds.close()
+    return {"workload": label, "max": mx, "mean": tot / max(px, 1), "pixels": px,
+            "frac_exact": exact / max(px, 1), "depth_and_ids_bit_exact": same}
+
+
+def plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode="graph"):
     """Render every (scene, viewports) pair resident in HBM two ways:
-    plain = one graph replay per scene (all its viewports), batched = one
-    graph replay for the whole set.  A frame is one viewport."""
+    plain = one graph replay per scene (all its viewports; plain_mode
+    "eager": one swegl_render call per scene instead), batched = one graph
+    replay for the whole set.  A frame is one viewport.  Each form carries
+    its HBM roofline: the jobs' algorithmic bytes over the measured time."""
     from swegl_amd.device import DeviceScene, Target
     from swegl_amd.host import SceneDescHolder
 
@@ -144,9 +201,11 @@ def plain_vs_batched(ctx, A, check, L, pairs, iters):
         jobs[k] = A.Job(ds[k].h, descs[k], len(descs[k]))
     n_frames = sum(len(d) for d in descs)
 
+    plain_fn = L.swegl_render_graph if plain_mode == "graph" else L.swegl_render
+
     def plain():
         for k in range(count):
-            check(L.swegl_render_graph(ctx.h, ds[k].h, descs[k], len(descs[k]), None), "render_graph")
+            check(plain_fn(ctx.h, ds[k].h, descs[k], len(descs[k]), None), "render_plain")
 
     def batched():
         check(L.swegl_render_batch_graph(ctx.h, jobs, count, None), "render_batch_graph")
@@ -167,7 +226,16 @@ def plain_vs_batched(ctx, A, check, L, pairs, iters):
         el = time.perf_counter() - a
         out[name] = {"frames_per_s": n_frames * iters / el, "gfragments_per_s": frags * iters / el / 1e9,
                      "ms_per_batch": el * 1e3 / iters}
+    out["plain"]["path"] = "graph replay per scene" if plain_mode == "graph" else "swegl_render per scene"
     out["batched_over_plain"] = out["batched"]["frames_per_s"] / out["plain"]["frames_per_s"]
+    # algorithmic bytes of the whole set (shaded pixels read back from the depth planes)
+    nbytes = 0
+    zk = iter(keep[1::2])
+    for s, vps in pairs:
+        nbytes += job_bytes(s, vps, [shaded_pixels(next(zk)) for _ in vps])
+    out["algorithmic_bytes_per_batch"] = nbytes
+    for name in ("plain", "batched"):
+        out[name]["roofline"] = roofline_of(nbytes, out[name]["ms_per_batch"] * 1e-3)
     for d in ds:
         d.close()
     return out
@@ -249,7 +317,7 @@ def bench_demo(ctx, A, check, L, seed, frames):
     return out, scene
 
 
-def bench_batch(ctx, A, check, L, count, seed, iters):
+def bench_batch(ctx, A, check, L, count, seed, iters, plain_mode="graph"):
     """Workload 'batch': `count` height fields (T log-uniform in [1e3, 2e5],
     square framebuffers of 256..1024, DoF on half)."""
     from swegl_amd import scenes
@@ -257,12 +325,12 @@ def bench_batch(ctx, A, check, L, count, seed, iters):
     t0 = time.perf_counter()
     pairs = scenes.batch_scenes(seed, count=count)
     gen_s = time.perf_counter() - t0
-    out = plain_vs_batched(ctx, A, check, L, pairs, iters)
+    out = plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode)
     out["generate_s"] = round(gen_s, 1)
     return out, pairs
 
 
-def bench_heavy(ctx, A, check, L, count, seed, iters):
+def bench_heavy(ctx, A, check, L, count, seed, iters, plain_mode="graph"):
     """Workload 'heavy': `count` scenes of 500 stacked 1000-triangle
     clusters (500,000 triangles), 4096^2 texture, sun + 8 point lights,
     each seen by 4 viewports of 1920x1080 with the 9-tap DoF."""
@@ -271,7 +339,7 @@ def bench_heavy(ctx, A, check, L, count, seed, iters):
     t0 = time.perf_counter()
     pairs = scenes.heavy_scenes(seed, count=count)
     gen_s = time.perf_counter() - t0
-    out = plain_vs_batched(ctx, A, check, L, pairs, iters)
+    out = plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode)
     out["generate_s"] = round(gen_s, 1)
     out["workload"] = (f"{count} scenes x 500,000 triangles (500 clusters of 1000, stacks of 8-20), "
                        "4 x 1920x1080 viewports each, tex 4096^2, sun + 8 point lights, 9-tap DoF r_max 8")
@@ -320,27 +388,36 @@ def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fr
         jobs[i] = A.Job(ds[k].h, C.pointer(descs[i]), 1)
     n_big = int((which == len(pairs)).sum())
 
-    # plain: every request alone, in arrival order (graphs captured first)
+    # plain: every request alone, in arrival order (graphs captured first);
+    # the warm-up render of each scene's first request gives its shaded
+    # pixels, which stand for every request of that scene in the texel term
+    fv = {}
     for k in range(len(allp)):
         i = int(np.argmax(which == k)) if (which == k).any() else None
         if i is not None:
             check(L.swegl_render_graph(ctx.h, ds[k].h, C.pointer(descs[i]), 1, None), "warm")
+            ctx.sync()
+            fv[k] = shaded_pixels(target(allp[k][1][0], i)[1])
     ctx.sync()
+    req_bytes = sum(job_bytes(allp[int(k)][0], allp[int(k)][1][:1], [fv[int(k)]]) for k in which)
     a = time.perf_counter()
     for i in range(n_req):
         check(L.swegl_render_graph(ctx.h, ds[int(which[i])].h, C.pointer(descs[i]), 1, None), "plain")
     ctx.sync()
     plain_fps = n_req / (time.perf_counter() - a)
 
+    plain_s = n_req / plain_fps
     out = {"requests": n_req, "large_requests": n_big, "lanes": lanes, "max_batch": max_batch,
-           "hbm_budget_bytes": 200 * 10**9, "plain_alone": {"frames_per_s": plain_fps}}
+           "hbm_budget_bytes": 200 * 10**9, "algorithmic_bytes": req_bytes,
+           "plain_alone": {"frames_per_s": plain_fps, "roofline": roofline_of(req_bytes, plain_s)}}
     srv = Server(ctx, 200 * 10**9, max_batch=max_batch, lanes=lanes)
     # saturated: everything queued at t=0
     lat = srv.run_trace(jobs, np.zeros(n_req))
     st = srv.stats()
     cap_fps = n_req / lat.max()
     out["saturated"] = {"frames_per_s": cap_fps, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-                        "p99_ms": float(np.percentile(lat, 99) * 1e3), "batches": st["batches"]}
+                        "p99_ms": float(np.percentile(lat, 99) * 1e3), "batches": st["batches"],
+                        "roofline": roofline_of(req_bytes, float(lat.max()))}
     out["served_over_plain"] = cap_fps / plain_fps
     rates = []
     for f in fracs:
@@ -379,6 +456,9 @@ def main():
     ap.add_argument("--requests", type=int, default=20000, help="served requests per rate (0 = skip)")
     ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--max-batch", type=int, default=64)
+    ap.add_argument("--plain", choices=("graph", "eager"), default="graph",
+                    help="plain form of the batch and heavy legs: one graph replay or one swegl_render per scene")
+    ap.add_argument("--reshade", type=int, default=1, help="FP64 re-shade error per workload (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also copy a target of this many MiB 3 times (PMC calibration)")
     args = ap.parse_args()
@@ -525,14 +605,15 @@ def main():
     if args.batch:
         if "batch" in tuning:
             ctx.apply_tuning(tuning["batch"])
-        out["batch"], pairs = bench_batch(ctx, A, check, L, args.batch, args.seed + 1, args.batch_iters)
+        out["batch"], pairs = bench_batch(ctx, A, check, L, args.batch, args.seed + 1, args.batch_iters,
+                                          args.plain)
         if args.requests:
             out["served"] = bench_served(ctx, A, check, L, pairs, (scene, vps), args.requests,
                                          args.seed + 2, args.lanes, args.max_batch, (0.3, 0.6, 0.9))
     if args.heavy:
         if "heavy" in tuning:
             ctx.apply_tuning(tuning["heavy"])
-        out["heavy"] = bench_heavy(ctx, A, check, L, args.heavy, args.seed + 3, args.heavy_iters)
+        out["heavy"] = bench_heavy(ctx, A, check, L, args.heavy, args.seed + 3, args.heavy_iters, args.plain)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, vps, args.cpu_seconds, args.cpu_threads)
         if cube is not None:
@@ -541,7 +622,30 @@ def main():
             from swegl_amd import scenes as _sc
             out["cpu_baseline_demo"] = cpu_baseline(demo_scene, [_sc.demo_viewport(layers=3)], 4.0,
                                                     args.cpu_threads)
-        out["fp64_reshade"] = fp64_reshade(ctx, A, check, L, ds, scene, vp)
+    if args.reshade:
+        e = fp64_reshade(ctx, A, check, L, ds, scene, vp)
+        e["workload"] = "heightfield 999,698 triangles, 1920x1080 (post shader off)"
+        per = {"heightfield": e}
+        if cube is not None:
+            per["cube"] = fp64_reshade_jobs(ctx, A, check, L, [cube], "cube 640x480")
+        if args.batch:
+            per["batch"] = fp64_reshade_jobs(ctx, A, check, L, pairs[:8], "first 8 scenes of the mixed batch")
+            # served requests are mixed-batch scenes and the large shape at per-request poses
+            rng = np.random.default_rng(args.seed + 4)
+            reqs = []
+            for s2, v2 in [pairs[8], pairs[9], pairs[10], (scene, vps)]:
+                keep_view = v2[0].m_camera.m_viewmatrix.copy()
+                v2[0].m_camera.rotate_y(float(rng.uniform(-0.3, 0.3)))
+                reqs.append((s2, v2[:1], keep_view))
+            per["served"] = fp64_reshade_jobs(ctx, A, check, L, [(a, b) for a, b, _ in reqs],
+                                              "3 mixed-batch requests and 1 large request at per-request poses")
+            for _, v2, kv in reqs:
+                v2[0].m_camera.m_viewmatrix[...] = kv
+        if args.heavy:
+            hv = scenes.heavy_scenes(args.seed + 3, count=1)
+            per["heavy"] = fp64_reshade_jobs(ctx, A, check, L, hv, "one heavy scene, its 4 viewports of 1920x1080")
+        worst = max(v["max"] for v in per.values())
+        out["fp64_reshade"] = dict(per["heightfield"], per_workload=per, max_over_workloads=worst)
     print(json.dumps(out))
 
 

@@ -5,6 +5,7 @@
 // layout compatibility only.
 #pragma once
 
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -112,7 +113,41 @@ struct animation_t {
     std::vector<animation_channel_t> channels;
 };
 
+namespace detail
+{
+// Each scene_t object gets an id of its own (constructed, copied or moved
+// into), so that device copies are never matched to another scene that
+// happens to reuse an address; the renderer installs the hook that frees a
+// scene's device copy when the scene object goes away.
+inline std::uint64_t next_scene_id()
+{
+    static std::atomic<std::uint64_t> n{0};
+    return ++n;
+}
+struct scene_hook {
+    void (*fn)(void *, std::uint64_t) = nullptr;   // trivially destructible: safe at exit
+    void *arg = nullptr;
+};
+inline scene_hook &scene_gone_hook()
+{
+    static scene_hook h;
+    return h;
+}
+}  // namespace detail
+
 struct scene_t {
+    scene_t() = default;
+    scene_t(const scene_t &o) = default;
+    scene_t(scene_t &&o) = default;
+    scene_t &operator=(const scene_t &o) = default;
+    scene_t &operator=(scene_t &&o) = default;
+    ~scene_t()
+    {
+        const detail::scene_hook h = detail::scene_gone_hook();
+        if (h.fn)
+            h.fn(h.arg, device_id.v);
+    }
+
     std::vector<node_t> nodes;
     std::vector<int> root_nodes;
     float ambient_light_intensity = 0;
@@ -123,9 +158,23 @@ struct scene_t {
     std::vector<material_t> materials;
     std::vector<texture_t> images;
     std::vector<animation_t> animations;
-    // Bump after editing vertices, indices or textures in place: render()
-    // re-uploads meshes and textures only when this changes.
+    // Bump after editing vertex, index, texture or material values in
+    // place: render() re-uploads meshes and textures when this changes, or
+    // when an array was resized, replaced or re-allocated (noticed by its
+    // address and size); node poses and lights are sent on every call.
     std::uint64_t device_generation = 0;
+    // Identity of this scene object for the renderer's device copy: a fresh
+    // id on construction and on every copy or move into it.
+    struct object_id {
+        std::uint64_t v = detail::next_scene_id();
+        object_id() = default;
+        object_id(const object_id &) : v(detail::next_scene_id()) {}
+        object_id &operator=(const object_id &)
+        {
+            v = detail::next_scene_id();
+            return *this;
+        }
+    } device_id;
 
     // scene_t::animate (model.hpp:146-177): poses the animated nodes; the
     // next render picks the poses up.

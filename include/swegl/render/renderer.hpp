@@ -36,16 +36,32 @@ struct device_state {
     struct scene_entry {
         swegl_scene *ds = nullptr;
         std::uint64_t gen = 0;
+        std::uint64_t shape = 0;   // scene_shape() at upload
     };
-    std::unordered_map<const scene_t *, scene_entry> scenes;
+    std::unordered_map<std::uint64_t, scene_entry> scenes;   // by scene_t::device_id
     struct surface_entry {
         swegl_target *t = nullptr;
         int w = 0, h = 0;
     };
     std::unordered_map<const SDL_Surface *, surface_entry> surfaces;
-    device_state() { check(swegl_ctx_create(0, &ctx), "swegl_ctx_create"); }
+    device_state()
+    {
+        check(swegl_ctx_create(0, &ctx), "swegl_ctx_create");
+        // a scene object that goes away takes its device copy with it
+        swegl::detail::scene_gone_hook() = {&device_state::scene_gone, this};
+    }
+    static void scene_gone(void *self, std::uint64_t id)
+    {
+        device_state &st = *(device_state *)self;
+        auto it = st.scenes.find(id);
+        if (it != st.scenes.end()) {
+            swegl_scene_destroy(st.ctx, it->second.ds);
+            st.scenes.erase(it);
+        }
+    }
     ~device_state()
     {
+        swegl::detail::scene_gone_hook() = {};
         for (auto &kv : scenes)
             swegl_scene_destroy(ctx, kv.second.ds);
         for (auto &kv : surfaces)
@@ -53,6 +69,40 @@ struct device_state {
         swegl_ctx_destroy(ctx);
     }
 };
+
+// A cheap signature of what an upload copies: node and primitive counts,
+// each primitive's vertex and index array addresses and sizes, each
+// texture's bitmap address and size.  A change (arrays grown, replaced or
+// re-allocated, textures added) triggers a re-upload by itself; edits of
+// values in place keep the signature and need device_generation bumped.
+inline std::uint64_t scene_shape(const scene_t &s)
+{
+    std::uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](std::uint64_t v) {
+        h ^= v;
+        h *= 1099511628211ull;
+    };
+    mix(s.nodes.size());
+    for (const node_t &n : s.nodes) {
+        mix(n.primitives.size());
+        for (const primitive_t &p : n.primitives) {
+            mix((std::uint64_t)(std::uintptr_t)p.vertices.data());
+            mix(p.vertices.size());
+            mix((std::uint64_t)(std::uintptr_t)p.indices.data());
+            mix(p.indices.size());
+            mix((std::uint64_t)p.mode);
+            mix((std::uint64_t)(std::int64_t)p.material_id);
+        }
+    }
+    mix(s.images.size());
+    for (const texture_t &t : s.images)
+        for (const auto &m : t.m_mipmaps) {
+            mix((std::uint64_t)(std::uintptr_t)m->m_bitmap.data());
+            mix(m->m_bitmap.size());
+        }
+    mix(s.materials.size());
+    return h;
+}
 
 inline device_state &state()
 {
@@ -178,14 +228,16 @@ inline swegl_target *surface_target(device_state &st, SDL_Surface *s)
 inline void render_impl(scene_t &scene, viewport_t *const *vps, size_t n)
 {
     device_state &st = state();
-    auto &se = st.scenes[&scene];
-    if (!se.ds || se.gen != scene.device_generation) {
+    auto &se = st.scenes[scene.device_id.v];
+    const std::uint64_t shape = scene_shape(scene);
+    if (!se.ds || se.gen != scene.device_generation || se.shape != shape) {
         if (se.ds)
             swegl_scene_destroy(st.ctx, se.ds);
         se.ds = nullptr;
         scene_arrays arrays(scene);
         check(swegl_scene_upload(st.ctx, &arrays.d, &se.ds), "swegl_scene_upload");
         se.gen = scene.device_generation;
+        se.shape = shape;
     } else {
         // meshes and textures are resident: send the poses and lights only
         scene_arrays arrays(scene, false);

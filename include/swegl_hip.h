@@ -304,11 +304,15 @@ int  swegl_render_batch_graph(swegl_ctx *ctx, const swegl_job *jobs, uint32_t n_
 /* ---- served: continuous batching ----------------------------------------- */
 /* Requests (a scene plus its viewports) are queued by swegl_server_submit.
  * swegl_server_poll retires finished batches and admits queued requests into
- * every free lane as one batched launch sequence, FIFO, as long as the bytes
- * of the requests in flight stay within the HBM budget.  Batches on
- * different lanes run concurrently, so requests that arrive while others are
- * on the GPU start as soon as a lane and memory free up.  Each result is
- * bit-identical to rendering the request alone. */
+ * every free lane as one batched launch sequence, FIFO, as long as the
+ * device bytes the server holds stay within the HBM budget: each lane's
+ * scratch region (kept between batches; idle lanes give theirs back when
+ * the queue's head would not fit otherwise) and the target planes of the
+ * requests in flight.  A request's scratch is bounded by the formulas the
+ * launch sizes its buffers with.  Batches on different lanes run
+ * concurrently, so requests that arrive while others are on the GPU start
+ * as soon as a lane and memory free up.  Each result is bit-identical to
+ * rendering the request alone. */
 typedef struct swegl_server_config {
     uint64_t hbm_budget_bytes;   /* admission budget for requests in flight     */
     uint32_t max_batch;          /* requests per launch sequence (0 = 256)       */
@@ -317,7 +321,8 @@ typedef struct swegl_server_config {
 
 typedef struct swegl_server_stats {
     uint64_t submitted, completed, batches;
-    uint64_t bytes_in_flight, peak_bytes_in_flight;
+    uint64_t bytes_in_flight;        /* scratch bounds + target planes of requests in flight */
+    uint64_t peak_bytes_in_flight;   /* most device bytes held at once (lane regions + targets) */
     uint32_t largest_batch;
 } swegl_server_stats;
 

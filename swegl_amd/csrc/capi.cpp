@@ -186,16 +186,19 @@ struct GraphEntry {
     }
 };
 
-// An execution lane: a stream with its own launch resources, upload staging
-// and (for lanes other than the context's) its own per-viewport scratch, so
-// that batches on different lanes can run concurrently.
+// An execution lane: a stream with its own launch resources and upload
+// staging.  The context's lane keeps per-viewport scratch with each scene;
+// a server lane (own_views) carves all of a batch's scratch -- per-viewport
+// buffers and launch resources -- from one grow-only region of its own, so
+// that batches on different lanes run concurrently and a lane holds exactly
+// one region whatever scenes pass through it.
 struct Lane {
     hipStream_t stream = nullptr;
     Resources res;
+    DevBuf region;
     StageSlot stage[4];
     int stage_next = 0;
     bool own_views = false;
-    std::map<std::pair<swegl_scene *, int>, std::unique_ptr<ViewScratch>> views;
     ViewScratch *scratch(swegl_scene *s, int use);
     int init(bool own);
     ~Lane();
@@ -229,8 +232,10 @@ struct swegl_ctx {
 
 // The graph cache keeps at most this many captured launch sequences and
 // this many bytes of their resources; the least recently replayed go first.
-static const size_t kGraphCacheEntries = 64;
-static const size_t kGraphCacheBytes = (size_t)4 << 30;
+// The limits hold a working set of a thousand launch shapes (a 256-scene
+// set replayed one scene at a time needs 256 entries and under 1 GiB).
+static const size_t kGraphCacheEntries = 1024;
+static const size_t kGraphCacheBytes = (size_t)16 << 30;
 
 // --------------------------------------------------------------- helpers
 static hipEvent_t take_event(swegl_ctx *c)
@@ -309,17 +314,12 @@ static int upload(Lane &L, void *dst, const void *src, size_t bytes)
     return SWEGL_OK;
 }
 
+// The context lane's scratch for the use-th viewport of scene s in a wave.
 ViewScratch *Lane::scratch(swegl_scene *s, int use)
 {
-    if (!own_views) {
-        while ((int)s->views.size() <= use)
-            s->views.emplace_back(new ViewScratch);
-        return s->views[use].get();
-    }
-    std::unique_ptr<ViewScratch> &p = views[{s, use}];
-    if (!p)
-        p.reset(new ViewScratch);
-    return p.get();
+    while ((int)s->views.size() <= use)
+        s->views.emplace_back(new ViewScratch);
+    return s->views[use].get();
 }
 
 int Lane::init(bool own)
@@ -1364,17 +1364,51 @@ static int no_waves(swegl_frame_stats *stats)
     return SWEGL_OK;
 }
 
-static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swegl_frame_stats *stats)
+// A server lane's batch: the scratch is measured by a pass over a region of
+// unbounded size at a dummy base, the lane's region grown to that size if
+// needed, then the batch prepared again on the region.
+static int prepare_in_region(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::vector<Wave> &waves,
+                             ResPtr &R)
 {
-    std::vector<Wave> waves;
-    int rc = prepare(c, L, all, waves);
+    Arena M;
+    M.base = (char *)(uintptr_t)256;   // never dereferenced: sizes only
+    M.cap = (size_t)1 << 62;
+    int rc = prepare(c, L, all, waves, &M);
     if (rc)
         return rc;
     if (waves.empty())
-        return no_waves(stats);
-    if ((rc = size_resources(L.res, waves)))
+        return SWEGL_OK;
+    if ((rc = carve_resources(M, waves, R)))
         return rc;
-    const ResPtr R = res_ptr(L.res);
+    if ((rc = L.region.ensure(M.used)))
+        return rc;
+    Arena A;
+    A.base = L.region.as<char>();
+    A.cap = L.region.n;
+    if ((rc = prepare(c, L, all, waves, &A)))
+        return rc;
+    return carve_resources(A, waves, R);
+}
+
+static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swegl_frame_stats *stats)
+{
+    std::vector<Wave> waves;
+    ResPtr R;
+    int rc;
+    if (L.own_views) {
+        if ((rc = prepare_in_region(c, L, all, waves, R)))
+            return rc;
+        if (waves.empty())
+            return no_waves(stats);
+    } else {
+        if ((rc = prepare(c, L, all, waves)))
+            return rc;
+        if (waves.empty())
+            return no_waves(stats);
+        if ((rc = size_resources(L.res, waves)))
+            return rc;
+        R = res_ptr(L.res);
+    }
     for (size_t w = 0; w < waves.size(); w++) {
         if ((rc = upload(L, R.jobs + waves[w].job_offset, waves[w].jd.data(),
                          waves[w].jd.size() * sizeof(JobDev))) ||
@@ -1646,7 +1680,7 @@ struct SrvReq {
     uint64_t ticket;
     swegl_scene *scene;
     std::vector<swegl_viewport_desc> vps;
-    uint64_t bytes;
+    uint64_t scratch, targets;   // request_bytes
 };
 
 struct swegl_server {
@@ -1657,12 +1691,16 @@ struct swegl_server {
         hipEvent_t done = nullptr;
         bool busy = false;
         std::vector<SrvReq> reqs;
-        uint64_t bytes = 0;
+        uint64_t scratch = 0, targets = 0;   // of the batch in flight
+        // device bytes the lane holds: its region (kept between batches),
+        // or while busy at least its batch's scratch bound
+        uint64_t held() const { return std::max<uint64_t>(lane->region.n, busy ? scratch : 0); }
     };
     std::vector<LaneState> lanes;
     std::deque<SrvReq> pending;
     std::deque<uint64_t> finished;
     uint64_t next_ticket = 1, in_flight = 0, peak = 0, submitted = 0, completed = 0, batches = 0;
+    uint64_t targets_in_flight = 0;
     uint32_t largest = 0;
     ~swegl_server()
     {
@@ -1675,18 +1713,35 @@ struct swegl_server {
     }
 };
 
-// Device bytes a request holds while in flight: per-viewport scratch, its
-// share of the bin arrays and the surfaces it writes.
-static uint64_t request_bytes(const swegl_scene *s, const std::vector<swegl_viewport_desc> &vps)
+// Device bytes of a request, split into the scratch its batch carves from
+// the lane's region and the target planes it writes.  The scratch is an
+// upper bound of the request's share of a batch's region, by the formulas
+// build_wave and resource_sizes use: per viewport the six scratch pieces
+// (each 256-byte aligned), its tiles' counters and offsets at the smallest
+// tile a wave can take (16x16, 8x8 with layers), its triangles' bins at
+// the wave's bin limit, its job table entry, plus a fixed share covering a
+// batch's minimum bin array, counter block and alignment.
+struct ReqBytes {
+    uint64_t scratch = 0, targets = 0;
+};
+static ReqBytes request_bytes(const swegl_ctx *c, const swegl_scene *s, const std::vector<swegl_viewport_desc> &vps)
 {
-    uint64_t b = 0;
+    ReqBytes b;
+    b.scratch = 16384 + (uint64_t)SW_CTR_BIG0 * 4 + 8 * 256;
     for (const swegl_viewport_desc &v : vps) {
-        const uint64_t V = s->n_vertices, T = s->n_tris, P = (uint64_t)v.w * v.h;
-        b += V * 17 + T * 12 + T * 16 * 4 + P * 8;
-        if (v.post_shader != SWEGL_POST_NULL)
-            b += P * 4;
-        if (v.triangle_id)
-            b += P * 4;
+        size_t sz[6];
+        job_scratch_sizes(s, v, sz);
+        for (size_t k : sz)
+            b.scratch += (k + 255) & ~(uint64_t)255;
+        const bool layers = v.transparency_layer_count > 0;
+        const int t = layers ? SW_LAYER_TW : 16;
+        const uint64_t tiles = (uint64_t)((v.w + t - 1) / t) * ((v.h + t - 1) / t);
+        const uint32_t bt = std::max<uint32_t>(1, layers ? std::max<uint32_t>(c->big_tiles, SW_LAYER_BIG_TILES)
+                                                          : c->big_tiles);
+        b.scratch += tiles * (SW_TILE_PAD * 4 + 4) + 16 + (uint64_t)std::max<uint32_t>(1, s->n_tris) * bt * 4 +
+                     sizeof(JobDev) + 4;
+        const uint64_t P = (uint64_t)v.w * v.h;
+        b.targets += (uint64_t)v.color->w * v.color->h * 4 + P * 4 + (v.triangle_id ? P * 4 : 0);
     }
     return b;
 }
@@ -1735,16 +1790,30 @@ extern "C" int swegl_server_submit(swegl_server *sv, const swegl_job *job, uint6
         if (rc)
             return rc;
     }
-    r.bytes = request_bytes(r.scene, r.vps);
-    if (r.bytes > sv->cfg.hbm_budget_bytes)
-        return fail(SWEGL_EAGAIN, "request needs %llu bytes, budget is %llu", (unsigned long long)r.bytes,
-                    (unsigned long long)sv->cfg.hbm_budget_bytes);
+    const ReqBytes rb = request_bytes(sv->c, r.scene, r.vps);
+    r.scratch = rb.scratch;
+    r.targets = rb.targets;
+    if (r.scratch + r.targets > sv->cfg.hbm_budget_bytes)
+        return fail(SWEGL_EAGAIN, "request needs %llu bytes, budget is %llu",
+                    (unsigned long long)(r.scratch + r.targets), (unsigned long long)sv->cfg.hbm_budget_bytes);
     r.ticket = sv->next_ticket++;
     if (ticket)
         *ticket = r.ticket;
     sv->pending.push_back(std::move(r));
     sv->submitted++;
     return SWEGL_OK;
+}
+
+// Device bytes the server holds with lane `l` taking a batch of `scratch`
+// and `targets` bytes: every lane's region (a busy lane at least its
+// batch's bound), lane l's at least the new batch's, and the target planes
+// of every request in flight.
+static uint64_t projected_bytes(const swegl_server *sv, size_t l, uint64_t scratch, uint64_t targets)
+{
+    uint64_t b = sv->targets_in_flight + targets;
+    for (size_t m = 0; m < sv->lanes.size(); m++)
+        b += m == l ? std::max<uint64_t>(sv->lanes[m].lane->region.n, scratch) : sv->lanes[m].held();
+    return b;
 }
 
 static int server_pump(swegl_server *sv)
@@ -1760,20 +1829,39 @@ static int server_pump(swegl_server *sv)
         for (const SrvReq &r : l.reqs)
             sv->finished.push_back(r.ticket);
         sv->completed += l.reqs.size();
-        sv->in_flight -= l.bytes;
+        sv->in_flight -= l.scratch + l.targets;
+        sv->targets_in_flight -= l.targets;
         l.reqs.clear();
-        l.bytes = 0;
+        l.scratch = l.targets = 0;
         l.busy = false;
     }
-    for (auto &l : sv->lanes) {
+    const uint64_t budget = sv->cfg.hbm_budget_bytes;
+    for (size_t li = 0; li < sv->lanes.size(); li++) {
+        auto &l = sv->lanes[li];
         if (l.busy || sv->pending.empty())
             continue;
-        uint64_t bytes = 0;
-        while (!sv->pending.empty() && l.reqs.size() < sv->cfg.max_batch &&
-               sv->in_flight + bytes + sv->pending.front().bytes <= sv->cfg.hbm_budget_bytes) {
-            bytes += sv->pending.front().bytes;
-            l.reqs.push_back(std::move(sv->pending.front()));
-            sv->pending.pop_front();
+        uint64_t scratch = 0, targets = 0;
+        for (int pass = 0; pass < 2 && l.reqs.empty(); pass++) {
+            while (!sv->pending.empty() && l.reqs.size() < sv->cfg.max_batch &&
+                   projected_bytes(sv, li, scratch + sv->pending.front().scratch,
+                                   targets + sv->pending.front().targets) <= budget) {
+                scratch += sv->pending.front().scratch;
+                targets += sv->pending.front().targets;
+                l.reqs.push_back(std::move(sv->pending.front()));
+                sv->pending.pop_front();
+            }
+            if (!l.reqs.empty() || pass)
+                break;
+            // the head does not fit: give back the regions idle lanes keep
+            bool freed = false;
+            for (auto &m : sv->lanes)
+                if (!m.busy && m.lane->region.n) {
+                    HIPCHK(hipStreamSynchronize(m.lane->stream));
+                    m.lane->region.release();
+                    freed = true;
+                }
+            if (!freed)
+                break;
         }
         if (l.reqs.empty())
             break;   // head of the queue waits for memory
@@ -1786,9 +1874,11 @@ static int server_pump(swegl_server *sv)
             return rc;
         HIPCHK(hipEventRecord(l.done, l.lane->stream));
         l.busy = true;
-        l.bytes = bytes;
-        sv->in_flight += bytes;
-        sv->peak = std::max(sv->peak, sv->in_flight);
+        l.scratch = scratch;
+        l.targets = targets;
+        sv->in_flight += scratch + targets;
+        sv->targets_in_flight += targets;
+        sv->peak = std::max(sv->peak, projected_bytes(sv, sv->lanes.size(), 0, 0));
         sv->batches++;
         sv->largest = std::max<uint32_t>(sv->largest, (uint32_t)l.reqs.size());
     }

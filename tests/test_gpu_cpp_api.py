@@ -57,3 +57,38 @@ def test_cpp_program_matches_oracle(tmp_path, seed):
     got = F()
     got.pixels, got.zbuffer, got.tid = got_px, got_z, None
     compare(got, want, colour_tol=1, check_tid=False)
+
+
+def test_cpp_scenes_at_a_reused_address_render_their_own_meshes():
+    """scene_t objects built one after another in one stack slot, each with
+    another quad, then one scene's index array replaced and its vertices
+    edited in place with a generation bump: every render's coverage is that
+    of its own geometry, as the oracle renders it."""
+    exe = os.path.join(HERE, "cpp", "scene_reuse")
+    assert os.path.exists(exe), "build() compiles tests/cpp/scene_reuse"
+    w, h = 160, 120
+    out = subprocess.run([exe, str(w), str(h)], check=True, timeout=120, capture_output=True, text=True)
+    got = [int(x) for x in out.stdout.split()]
+
+    from swegl_amd.host import node_t, pixel_shader_t, primitive_t
+    from swegl_amd import _abi as A
+
+    def expect(half, indices, scale=1.0):
+        xs, ys = [-half, half, -half, half], [-half, -half, half, half]
+        pos = np.array([[x * scale, y * scale, 0] for x, y in zip(xs, ys)], np.float32)
+        prim = primitive_t(pos, np.tile([0, 0, 1], (4, 1)), np.full((4, 2), 0.5), indices, A.TRIANGLES, 0)
+        s = scene_t()
+        s.materials = [material_t(pixel_colors(200, 100, 50, 255), 1, 1, -1, False)]
+        s.nodes, s.root_nodes = [node_t([prim])], [0]
+        s.ambient_light_intensity = 1.0
+        vp = viewport_t(0, 0, w, h, Surface(w, h), pixel_shader_t(), 0)
+        vp.m_camera.translate(0, 0, -3)
+        vp.set_post_shader(post_shader_t())
+        fr = oracle.render(s, [vp])[0]
+        return int((fr.zbuffer.view(np.uint32) != 0x7F7F7F7F).sum())
+
+    quad = [0, 1, 2, 2, 1, 3]
+    want = [expect(0.25, quad), expect(0.5, quad), expect(1.0, quad), expect(1.0, [0, 1, 2]),
+            expect(1.0, [0, 1, 2], scale=0.5)]
+    assert len(set(want)) == 5
+    assert got == want
