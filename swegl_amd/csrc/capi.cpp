@@ -1213,7 +1213,7 @@ static void resource_sizes(std::vector<Wave> &waves, size_t sz[5])
     }
     sz[0] = job_bytes;
     sz[1] = (size_t)tiles * SW_TILE_PAD * 4 + 8;
-    sz[2] = (size_t)tiles * 4 + 8;
+    sz[2] = (size_t)tiles * 8 + 8;   // offsets [0, tiles], then the claim order
     sz[3] = cap * 4;
     sz[4] = (size_t)n_ctr * 4;
 }
@@ -1260,6 +1260,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bo
     B.n_tiles = W.tiles;
     B.tile_count = R.tile_count;
     B.tile_offset = R.tile_offset;
+    B.tile_order = R.tile_offset + W.tiles + 1;
     B.tile_cursor = R.tile_count + 1;
     B.bins = R.bins;
     B.bin_capacity = (uint32_t)W.cap;
@@ -1278,9 +1279,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bo
             return fail(SWEGL_EIO, "%s launch: %s", name, hipGetErrorString(e_));       \
     } while (0)
     STAGE("vertex", launch_vertex(B, W.vblk, zero0, W.n_ctr - zero0, st));
-    STAGE("mark", launch_mark(B, W.tblk, st));
-    STAGE("bin_count", launch_bin_count(B, W.tblk, st));
-    STAGE("scan", launch_scan(B, st));
+    STAGE("front", launch_front(B, W.tblk, st));
     STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
     if (W.layers) {
         const uint32_t pixels = (uint32_t)(W.jd[0].V.vw * W.jd[0].V.vh);
@@ -1738,7 +1737,7 @@ static ReqBytes request_bytes(const swegl_ctx *c, const swegl_scene *s, const st
         const uint64_t tiles = (uint64_t)((v.w + t - 1) / t) * ((v.h + t - 1) / t);
         const uint32_t bt = std::max<uint32_t>(1, layers ? std::max<uint32_t>(c->big_tiles, SW_LAYER_BIG_TILES)
                                                           : c->big_tiles);
-        b.scratch += tiles * (SW_TILE_PAD * 4 + 4) + 16 + (uint64_t)std::max<uint32_t>(1, s->n_tris) * bt * 4 +
+        b.scratch += tiles * (SW_TILE_PAD * 4 + 8) + 16 + (uint64_t)std::max<uint32_t>(1, s->n_tris) * bt * 4 +
                      sizeof(JobDev) + 4;
         const uint64_t P = (uint64_t)v.w * v.h;
         b.targets += (uint64_t)v.color->w * v.color->h * 4 + P * 4 + (v.triangle_id ? P * 4 : 0);

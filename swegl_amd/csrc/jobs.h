@@ -121,6 +121,7 @@ struct BatchDev {
     uint32_t n_tiles;         // total tiles over all jobs
     uint32_t *tile_count;     // [n_tiles * SW_TILE_PAD], see SW_TILE_PAD
     uint32_t *tile_offset;    // [n_tiles + 1]
+    uint32_t *tile_order;     // [n_tiles]: tiles by descending bin count (k_scan), the raster's claim order
     uint32_t *tile_cursor;    // tile_count + 1: the fill ranks
     uint32_t *bins;           // [capacity]
     uint32_t bin_capacity;
@@ -139,7 +140,12 @@ struct BatchDev {
 #define SW_RASTER_QUEUES 8
 #define SW_CTR_QUEUE0 SW_CTR_CONTROL                              // queue k at + 64k (own 256 B)
 #define SW_CTR_TOTAL (SW_CTR_CONTROL + 64 * SW_RASTER_QUEUES)
-#define SW_CTR_BIG0 (SW_CTR_TOTAL + 64)
+// k_front's finished blocks: SW_TICKET_GROUPS counters of the blocks with
+// index k mod SW_TICKET_GROUPS, then one of the finished groups, each on
+// its own 128-byte line (same-address atomics serialise where they execute)
+#define SW_TICKET_GROUPS 32
+#define SW_CTR_TICKETS (SW_CTR_TOTAL + 64)
+#define SW_CTR_BIG0 (SW_CTR_TICKETS + 32 * (SW_TICKET_GROUPS + 1))
 // statistic indices within a slot
 #define SW_ST_FRAGMENTS 0
 #define SW_ST_NAN 1

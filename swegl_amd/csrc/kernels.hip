@@ -2,8 +2,8 @@
 //
 // Stage map (reference loop -> kernel):
 //   vertex_shaders.hpp:16-71 (v_world, normal_world, frustum coords)  -> k_vertex
-//   renderer.cpp:83-185 (visibility marks)                           -> k_mark
-//   renderer.cpp:240-460 (clip, DDA setup) for binning              -> k_bin_count / k_bin_fill
+//   renderer.cpp:83-185 (visibility marks), binning counts          -> k_front
+//   renderer.cpp:240-460 (clip, DDA setup) for binning              -> k_front / k_bin_fill
 //   renderer.cpp:462-558 (spans, z test) + pixel shading            -> k_raster (per tile)
 //   post_shaders.hpp:51-131 (depth of field)                        -> k_dof
 // Every kernel works on a batch of jobs (one job = one viewport of one
@@ -170,24 +170,11 @@ __global__ __launch_bounds__(256) void k_vertex(BatchDev B, uint32_t w0, uint32_
     J.yes[i] = 0;
 }
 
-// -------------------------------------------------------------- k_mark
-// renderer.cpp:83-185: a vertex is marked when any triangle using it passes
-// the frustum test and (double-sided or front-facing) in frustum space.
-__global__ __launch_bounds__(256) void k_mark(BatchDev B, const JobDev *__restrict__ jobs)
+// A triangle's three vertices carry visibility marks (renderer.cpp:248-253
+// draws a triangle only then).
+__device__ __forceinline__ bool marked(const JobDev &J, uint32_t t)
 {
-    const JobDev &J = jobs[find_job<JT>(B, jobs, blockIdx.x)];
-    uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
-    if (t >= J.n_tris)
-        return;
-    uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
-    f3 a = ld3(J.ndc, i0), b = ld3(J.ndc, i1), c = ld3(J.ndc, i2);
-    if (!inside_frustum(a, b, c))
-        return;
-    if (!ro(J.prims)[prim_of_tri(J, t)].double_sided && !cross_z_sign(b - a, c - a, 1))
-        return;
-    J.yes[i0] = 1;
-    J.yes[i1] = 1;
-    J.yes[i2] = 1;
+    return ro(J.yes)[ro(J.tri)[3 * t]] && ro(J.yes)[ro(J.tri)[3 * t + 1]] && ro(J.yes)[ro(J.tri)[3 * t + 2]];
 }
 
 // ------------------------------------------------ fill_triangle on device
@@ -309,22 +296,18 @@ __device__ bool sub_extent(const Setup &S, const ViewXf &V, Extent &E)
 
 #define RECT_EMPTY 0xFFFFFFFFu
 
-// ---------------------------------------------------------- k_bin_count
-// Tile rectangle of every triangle, conservative but never too small: rows
-// are exact (ceil of vertex y, clipped to the viewport), columns are the
-// vertex x range widened by the DDA's worst-case accumulated rounding.
+// ------------------------------------------------------ tile rectangles
+// Tile rectangle of a triangle, conservative but never too small: rows are
+// exact (ceil of vertex y, clipped to the viewport), columns are the vertex
+// x range widened by the DDA's worst-case accumulated rounding.
 // Near-clipped or non-finite triangles take the exact fill_triangle path.
-// Each block of 256 consecutive triangles publishes the union rectangle and
-// counts one bin entry per tile it covers.
-__device__ bool quick_extent(const JobDev &J, uint32_t t, Extent &E, uint32_t &drawn, bool &exact)
+// The extent of an unclipped, finite triangle from its screen points; sets
+// `exact` when the triangle needs fill_triangle's exact path (near-clipped
+// or non-finite).  `drawn`: fill_triangle_2 would draw it (y0 != y2).
+__device__ bool quick_extent(const JobDev &J, f3 a, f3 b, f3 c, Extent &E, uint32_t &drawn, bool &exact)
 {
     exact = false;
-    uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
-    if (!(ro(J.yes)[i0] && ro(J.yes)[i1] && ro(J.yes)[i2]))
-        return false;
-    f3 a = frustum_to_screen(J.V, ld3(J.ndc, i0));
-    f3 b = frustum_to_screen(J.V, ld3(J.ndc, i1));
-    f3 c = frustum_to_screen(J.V, ld3(J.ndc, i2));
+    drawn = 0;
     const float big = 1073741824.0f;
     bool finite = fabsf(a.x) < big && fabsf(b.x) < big && fabsf(c.x) < big && fabsf(a.y) < big &&
                   fabsf(b.y) < big && fabsf(c.y) < big;
@@ -351,50 +334,214 @@ __device__ bool quick_extent(const JobDev &J, uint32_t t, Extent &E, uint32_t &d
     return E.x0 <= E.x1;
 }
 
-__global__ __launch_bounds__(256) void k_bin_count(BatchDev B, const JobDev *__restrict__ jobs)
+// The exact path: the union extent of fill_triangle's sub-triangles and how
+// many of them fill_triangle_2 draws.  Marks are not looked at here.
+__device__ bool exact_extent(const JobDev &J, uint32_t t, Extent &E, uint32_t &drawn)
+{
+    SubTri st[2];
+    const int ns = tri_subs<true>(J, t, st);
+    int ya = INT32_MAX, yb = INT32_MIN, xa = INT32_MAX, xb = INT32_MIN;
+    drawn = 0;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        Setup S;
+        if (s >= ns || !setup_subtri(st[s], J.V, S))
+            continue;
+        drawn++;
+        Extent e;
+        if (!sub_extent(S, J.V, e))
+            continue;
+        ya = min(ya, e.y0);
+        yb = max(yb, e.y1);
+        xa = min(xa, e.x0);
+        xb = max(xb, e.x1);
+    }
+    E.y0 = ya;
+    E.y1 = yb;
+    E.x0 = xa;
+    E.x1 = xb;
+    return ya < yb && xa <= xb;
+}
+
+// The tile rows of the k_front/k_bin_fill rectangle, shared by both.
+__device__ __forceinline__ uint2 tile_rect(const JobDev &J, const Extent &E, int &tx0, int &tx1, int &ty0, int &ty1)
+{
+    // tile edges are powers of two and the offsets are not negative: shifts
+    const int sx = __builtin_ctz((unsigned)J.tw), sy = __builtin_ctz((unsigned)J.th);
+    tx0 = (E.x0 - J.V.vx) >> sx;
+    tx1 = (E.x1 - J.V.vx) >> sx;
+    ty0 = (E.y0 - J.V.vy) >> sy;
+    ty1 = (E.y1 - 1 - J.V.vy) >> sy;
+    return make_uint2((uint32_t)tx0 | ((uint32_t)ty0 << 16), (uint32_t)tx1 | ((uint32_t)ty1 << 16));
+}
+
+// ------------------------------------------------------ tile count scan
+// Exclusive scan of the tile counts by one block, in passes of NT x
+// SCAN_PER tiles: each thread loads its SCAN_PER consecutive counts at once
+// (one memory round trip per pass), the block scans the per-thread sums
+// (shuffles within each wave, then the wave totals), and each thread writes
+// its tiles' offsets (densely: the fill ranks, zeroed by k_vertex, sit in
+// the padded lines and are added to the offsets by k_bin_fill).
+#define SCAN_PER 16
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d);
+        if (lane >= d)
+            v += o;
+    }
+    return v;
+}
+
+// Claim order of the raster's tiles: largest bin counts first (a counting
+// sort on a logarithmic bucket of the count, four buckets per power of
+// two), so that the costliest tiles start first and the last ones to start
+// are cheap -- the persistent raster's makespan then comes close to its
+// mean load instead of the heaviest tile started last.
+#define SW_ORDER_BUCKETS 128
+static_assert(SW_ORDER_BUCKETS == 2 * 64, "the bucket scan takes two buckets per lane of one wave");
+__device__ __forceinline__ uint32_t count_bucket(uint32_t c)
+{
+    if (c < 4)
+        return c;
+    const uint32_t e = 31u - (uint32_t)__clz(c);   // >= 2
+    return 4u * (e - 1u) + ((c >> (e - 2u)) & 3u);   // <= 123
+}
+
+// SCAN_PER consecutive tile counts from tile i0 on (zero past n), read with
+// device-scope atomic loads (the counts were added at the memory side in
+// this same kernel: an XCD's L2 may hold older lines), all issued before
+// the first is used.
+__device__ __forceinline__ void load_counts(const BatchDev &B, uint32_t i0, uint32_t n, uint32_t (&v)[SCAN_PER])
+{
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++)
+        v[k] = i0 + k < n ? __hip_atomic_load(&B.tile_count[(i0 + k) * SW_TILE_PAD], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT)
+                          : 0u;
+}
+
+// The scan and the claim order, by one block of NT threads.
+template <int NT>
+__device__ void scan_tiles(const BatchDev &B)
+{
+    constexpr int NW = NT / 64;
+    __shared__ uint32_t wsum[NW];
+    __shared__ uint32_t s_carry;
+    __shared__ uint32_t s_hist[SW_ORDER_BUCKETS];
+    const uint32_t n = B.n_tiles;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0)
+        s_carry = 0;
+    for (int b = threadIdx.x; b < SW_ORDER_BUCKETS; b += NT)
+        s_hist[b] = 0;
+    __syncthreads();
+    uint32_t v[SCAN_PER];
+    for (uint32_t base = 0; base < n; base += (uint32_t)NT * SCAN_PER) {
+        const uint32_t i0 = base + threadIdx.x * SCAN_PER;
+        load_counts(B, i0, n, v);
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++) {
+            sum += v[k];
+            if (i0 + k < n)
+                atomicAdd(&s_hist[count_bucket(v[k])], 1u);
+        }
+        const uint32_t incl = wave_incl_scan(sum);
+        if (lane == 63)
+            wsum[wave] = incl;
+        __syncthreads();
+        if (wave == 0) {
+            const uint32_t w = lane < NW ? wsum[lane] : 0u;
+            const uint32_t wi = wave_incl_scan(w);
+            if (lane < NW)
+                wsum[lane] = wi - w;   // exclusive prefix of the wave totals
+        }
+        __syncthreads();
+        const uint32_t carry = s_carry;
+        uint32_t run = carry + wsum[wave] + incl - sum;
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++) {
+            if (i0 + k < n) {
+                B.tile_offset[i0 + k] = run;
+                run += v[k];
+            }
+        }
+        __syncthreads();   // everyone has read s_carry and wsum
+        if (threadIdx.x == NT - 1)
+            s_carry = run;   // the last thread's running sum ends the pass
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        B.tile_offset[n] = s_carry;
+        B.ctr[SW_CTR_TOTAL] = s_carry;   // total bin entries
+    }
+    // first place of each bucket, the largest bucket first: wave 0 scans the
+    // buckets in descending order, two per lane
+    if (wave == 0) {
+        const int b0 = SW_ORDER_BUCKETS - 1 - 2 * lane, b1 = b0 - 1;
+        const uint32_t h0 = s_hist[b0], h1 = s_hist[b1];
+        const uint32_t incl = wave_incl_scan(h0 + h1);
+        s_hist[b0] = incl - h0 - h1;
+        s_hist[b1] = incl - h1;
+    }
+    __syncthreads();
+    // every tile takes the next place of its bucket (order within a bucket
+    // is arbitrary: the raster's result does not depend on the claim order);
+    // with one pass the counts are still in registers
+    for (uint32_t base = 0; base < n; base += (uint32_t)NT * SCAN_PER) {
+        const uint32_t i0 = base + threadIdx.x * SCAN_PER;
+        if (n > (uint32_t)NT * SCAN_PER)
+            load_counts(B, i0, n, v);
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++)
+            if (i0 + k < n)
+                B.tile_order[atomicAdd(&s_hist[count_bucket(v[k])], 1u)] = i0 + k;
+    }
+}
+
+// ------------------------------------------------------------- k_front
+// Per triangle, in one pass: the visibility marks (renderer.cpp:83-185: a
+// triangle passing the frustum test and double-sided or front-facing marks
+// its three vertices), its tile rectangle (conservative but never too small:
+// rows exact from ceil of vertex y, clipped to the viewport; columns the
+// vertex x range widened by the DDA's worst-case accumulated rounding;
+// near-clipped or non-finite triangles take the exact fill_triangle path)
+// and one bin count per covered tile.  Whether a triangle is drawn depends
+// on marks other triangles set in this same pass, so every triangle with a
+// rectangle is counted here; k_bin_fill, which runs after all marks are
+// set, places only triangles whose three vertices are marked, and the
+// raster checks the marks of big-list triangles.  The last block to finish
+// scans the tile counts and writes the raster's claim order.
+__global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restrict__ jobs)
 {
     const int jidx = find_job<JT>(B, jobs, blockIdx.x);
     const JobDev &J = jobs[jidx];
     const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
-    uint32_t drawn = 0, big = 0;
+    uint32_t big = 0;
     int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
     if (t < J.n_tris) {
+        const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
+        const f3 a = ld3(J.ndc, i0), b = ld3(J.ndc, i1), c = ld3(J.ndc, i2);
+        if (inside_frustum(a, b, c) &&
+            (ro(J.prims)[prim_of_tri(J, t)].double_sided || cross_z_sign(b - a, c - a, 1))) {
+            J.yes[i0] = 1;
+            J.yes[i1] = 1;
+            J.yes[i2] = 1;
+        }
         Extent E;
         bool exact;
-        bool have = quick_extent(J, t, E, drawn, exact);
-        if (exact) {
-            SubTri st[2];
-            int ns = tri_subs(J, t, st);
-            int ya = INT32_MAX, yb = INT32_MIN, xa = INT32_MAX, xb = INT32_MIN;
-#pragma unroll
-            for (int s = 0; s < 2; s++) {
-                Setup S;
-                if (s >= ns || !setup_subtri(st[s], J.V, S))
-                    continue;
-                drawn++;
-                Extent e;
-                if (!sub_extent(S, J.V, e))
-                    continue;
-                ya = min(ya, e.y0);
-                yb = max(yb, e.y1);
-                xa = min(xa, e.x0);
-                xb = max(xb, e.x1);
-            }
-            have = ya < yb && xa <= xb;
-            E.y0 = ya;
-            E.y1 = yb;
-            E.x0 = xa;
-            E.x1 = xb;
-        }
+        uint32_t drawn;
+        bool have = quick_extent(J, frustum_to_screen(J.V, a), frustum_to_screen(J.V, b),
+                                 frustum_to_screen(J.V, c), E, drawn, exact);
+        if (exact)
+            have = exact_extent(J, t, E, drawn);
         uint2 r = make_uint2(RECT_EMPTY, 0);
         if (have) {
-            // tile edges are powers of two and the offsets are not negative: shifts
-            const int sx = __builtin_ctz((unsigned)J.tw), sy = __builtin_ctz((unsigned)J.th);
-            tx0 = (E.x0 - J.V.vx) >> sx;
-            tx1 = (E.x1 - J.V.vx) >> sx;
-            ty0 = (E.y0 - J.V.vy) >> sy;
-            const int ty1 = (E.y1 - 1 - J.V.vy) >> sy;
-            r = make_uint2((uint32_t)tx0 | ((uint32_t)ty0 << 16), (uint32_t)tx1 | ((uint32_t)ty1 << 16));
+            int ty1;
+            r = tile_rect(J, E, tx0, tx1, ty0, ty1);
             ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
             if (ntiles > (int)B.big_tiles) {
                 ntiles = 0;
@@ -410,6 +557,75 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B, const JobDev *__r
             J.big[pos] = t;
     }
     // one aggregated atomic per distinct tile per wave and round
+    for (int k = 0; __any(k < ntiles); k++) {
+        const bool act = k < ntiles;
+        uint32_t key = 0;
+        if (act) {
+            const int w = tx1 - tx0 + 1;
+            const int ky = k == 0 ? 0 : k / w, kx = k == 0 ? 0 : k % w;   // k is the same in every lane
+            key = J.tile_base + (ty0 + ky) * J.tiles_x + (tx0 + kx);
+        }
+        wave_add<false>(B.tile_count, key * SW_TILE_PAD, act);
+    }
+    // The last block to finish scans.  The counts are device-scope atomics,
+    // performed at the memory side, not in an XCD's L2: each wave waits
+    // for its own to complete before the block takes its ticket, and the
+    // scan reads the counts with device-scope atomic loads.  (A device-scope
+    // fence here would write back and invalidate every XCD's L2 per block.)
+    // Tickets are taken in two levels (the blocks of group k = index mod
+    // SW_TICKET_GROUPS, then the groups), so that no counter sees more than
+    // about gridDim / SW_TICKET_GROUPS atomics.
+    __shared__ uint32_t s_last;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t G = gridDim.x, k = blockIdx.x % SW_TICKET_GROUPS;
+        const uint32_t in_group = (G - 1u - k) / SW_TICKET_GROUPS + 1u;   // blocks k, k + 32, ... below G
+        const uint32_t groups = G < SW_TICKET_GROUPS ? G : SW_TICKET_GROUPS;
+        uint32_t last = 0;
+        if (atomicAdd(B.ctr + SW_CTR_TICKETS + 32 * k, 1u) == in_group - 1u)
+            last = atomicAdd(B.ctr + SW_CTR_TICKETS + 32 * SW_TICKET_GROUPS, 1u) == groups - 1u ? 1u : 0u;
+        s_last = last;
+    }
+    __syncthreads();
+    if (!s_last)
+        return;
+    scan_tiles<256>(B);
+}
+
+// ----------------------------------------------------------- k_bin_fill
+__global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__restrict__ jobs)
+{
+    const JobDev &J = jobs[find_job<JT>(B, jobs, blockIdx.x)];
+    const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
+    int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
+    uint32_t drawn = 0, big = 0;
+    if (t < J.n_tris) {
+        const uint2 r = ro2(J.rect, t);
+        const bool m = (r.x != RECT_EMPTY || B.stats) && marked(J, t);   // every mark is set by now
+        if (r.x != RECT_EMPTY && m) {
+            tx0 = r.x & 0xFFFF;
+            ty0 = r.x >> 16;
+            tx1 = r.y & 0xFFFF;
+            const int ty1 = r.y >> 16;
+            ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+            if (ntiles > (int)B.big_tiles) {
+                ntiles = 0;
+                big = 1;
+            }
+        }
+        if (B.stats && m) {
+            // drawn sub-triangles (fill_triangle_2 calls that find a scanline)
+            const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
+            Extent E;
+            bool exact;
+            quick_extent(J, frustum_to_screen(J.V, ld3(J.ndc, i0)), frustum_to_screen(J.V, ld3(J.ndc, i1)),
+                         frustum_to_screen(J.V, ld3(J.ndc, i2)), E, drawn, exact);
+            if (exact)
+                exact_extent(J, t, E, drawn);
+        }
+    }
     uint32_t entries = 0;
     for (int k = 0; __any(k < ntiles); k++) {
         const bool act = k < ntiles;
@@ -420,7 +636,13 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B, const JobDev *__r
             key = J.tile_base + (ty0 + ky) * J.tiles_x + (tx0 + kx);
             entries++;
         }
-        wave_add<false>(B.tile_count, key * SW_TILE_PAD, act);
+        // the entry's place: the tile's offset (read while the rank's atomic is
+        // in flight) plus its rank among the tile's entries so far
+        const uint32_t off = act ? ro(B.tile_offset)[key] : 0u;
+        const uint32_t rank = wave_add<true>(B.tile_cursor, key * SW_TILE_PAD, act);
+        const uint32_t pos = off + rank;
+        if (act && pos < B.bin_capacity)
+            B.bins[pos] = t;
     }
     if (!B.stats)
         return;
@@ -442,111 +664,6 @@ __global__ __launch_bounds__(256) void k_bin_count(BatchDev B, const JobDev *__r
             stat_add(B, SW_ST_BINS, s_acc[1]);
         if (s_acc[2])
             stat_add(B, SW_ST_BIG, s_acc[2]);
-    }
-}
-
-// ----------------------------------------------------------- k_scan
-// Exclusive scan of the tile counts by one block of 1024 threads, in passes
-// of 1024 x SCAN_PER tiles: each thread loads its SCAN_PER consecutive
-// counts at once (one memory round trip per pass), the block scans the
-// per-thread sums (shuffles within each wave, then the 16 wave totals), and
-// each thread writes its tiles' offsets (densely: the fill ranks, zeroed by
-// k_vertex, sit in the padded lines and are added to the offsets by
-// k_bin_fill).
-#define SCAN_PER 8
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
-{
-    const int lane = __lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(v, d);
-        if (lane >= d)
-            v += o;
-    }
-    return v;
-}
-
-__global__ __launch_bounds__(1024) void k_scan(BatchDev B)
-{
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t s_carry;
-    const uint32_t n = B.n_tiles;
-    const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    if (threadIdx.x == 0)
-        s_carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < n; base += 1024u * SCAN_PER) {
-        const uint32_t i0 = base + threadIdx.x * SCAN_PER;
-        uint32_t v[SCAN_PER], sum = 0;
-#pragma unroll
-        for (int k = 0; k < SCAN_PER; k++) {
-            v[k] = i0 + k < n ? B.tile_count[(i0 + k) * SW_TILE_PAD] : 0u;
-            sum += v[k];
-        }
-        const uint32_t incl = wave_incl_scan(sum);
-        if (lane == 63)
-            wsum[wave] = incl;
-        __syncthreads();
-        if (wave == 0) {
-            const uint32_t w = lane < 16 ? wsum[lane] : 0u;
-            const uint32_t wi = wave_incl_scan(w);
-            if (lane < 16)
-                wsum[lane] = wi - w;   // exclusive prefix of the wave totals
-        }
-        __syncthreads();
-        const uint32_t carry = s_carry;
-        uint32_t run = carry + wsum[wave] + incl - sum;
-#pragma unroll
-        for (int k = 0; k < SCAN_PER; k++) {
-            if (i0 + k < n) {
-                B.tile_offset[i0 + k] = run;
-                run += v[k];
-            }
-        }
-        __syncthreads();   // everyone has read s_carry and wsum
-        if (threadIdx.x == 1023)
-            s_carry = run;   // the last thread's running sum ends the pass
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        B.tile_offset[n] = s_carry;
-        B.ctr[SW_CTR_TOTAL] = s_carry;   // total bin entries
-    }
-}
-
-// ----------------------------------------------------------- k_bin_fill
-__global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__restrict__ jobs)
-{
-    const JobDev &J = jobs[find_job<JT>(B, jobs, blockIdx.x)];
-    const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
-    int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
-    if (t < J.n_tris) {
-        const uint2 r = ro2(J.rect, t);
-        if (r.x != RECT_EMPTY) {
-            tx0 = r.x & 0xFFFF;
-            ty0 = r.x >> 16;
-            tx1 = r.y & 0xFFFF;
-            const int ty1 = r.y >> 16;
-            ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-            if (ntiles > (int)B.big_tiles)
-                ntiles = 0;
-        }
-    }
-    for (int k = 0; __any(k < ntiles); k++) {
-        const bool act = k < ntiles;
-        uint32_t key = 0;
-        if (act) {
-            const int w = tx1 - tx0 + 1;
-            const int ky = k == 0 ? 0 : k / w, kx = k == 0 ? 0 : k % w;   // k is the same in every lane
-            key = J.tile_base + (ty0 + ky) * J.tiles_x + (tx0 + kx);
-        }
-        // the entry's place: the tile's offset (read while the rank's atomic is
-        // in flight) plus its rank among the tile's entries so far
-        const uint32_t off = act ? ro(B.tile_offset)[key] : 0u;
-        const uint32_t rank = wave_add<true>(B.tile_cursor, key * SW_TILE_PAD, act);
-        const uint32_t pos = off + rank;
-        if (act && pos < B.bin_capacity)
-            B.bins[pos] = t;
     }
 }
 
@@ -823,7 +940,9 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
     // are dealt to the lanes, so a large triangle does not leave one lane
     // looping over its pixels while the others idle.
     __shared__ uint32_t s_tri[256], s_rows[256], s_first[256], s_wsum[4];
-    const uint32_t nb = B.tile_count[g * SW_TILE_PAD], off = B.tile_offset[g];
+    // entries placed by k_bin_fill (marked triangles only; the count also
+    // held unmarked ones), then the job's big list
+    const uint32_t nb = B.tile_cursor[g * SW_TILE_PAD], off = B.tile_offset[g];
     const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
     if (nb + nbig > 256) {
@@ -836,7 +955,7 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
                 t = ro(J.big)[e - nb];
                 const uint2 r = ro2(J.rect, t);
                 if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
-                    ty > (int)(r.y >> 16))
+                    ty > (int)(r.y >> 16) || !marked(J, t))
                     continue;
             }
             raster_depth(J, t, tr, tr.y0, tr.y1, keys, TW, frags, nans);
@@ -853,7 +972,7 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
                 t = ro(J.big)[e - nb];
                 const uint2 r = ro2(J.rect, t);
                 if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
-                    ty > (int)(r.y >> 16))
+                    ty > (int)(r.y >> 16) || !marked(J, t))
                     t = 0xFFFFFFFFu;
             }
         }
@@ -971,14 +1090,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ uint32_t s_tile;
     if (threadIdx.x < 3)
         s_cnt[threadIdx.x] = 0;
-    // Tiles come from SW_RASTER_QUEUES queues, each owning a contiguous run
-    // of tiles and counting on its own cache line: same-address atomics are
-    // serialised where they execute, so one shared queue would make every
+    // Tiles come from SW_RASTER_QUEUES queues over the claim order
+    // (B.tile_order, largest bin counts first), queue k owning the places
+    // k, k + 8, k + 16, ... so that every queue runs from costly tiles to
+    // cheap ones, each counting on its own cache line: same-address atomics
+    // are serialised where they execute, so one shared queue would make every
     // claim wait for all others.  A workgroup drains its home queue (block
-    // index mod 8, which is also the XCD the hardware deals consecutive
-    // workgroups to, so neighbouring tiles share an L2), then the others in
-    // turn; a queue seen exhausted is never asked again, so every workgroup
-    // stops after at most n_tiles + 8 claims in total.
+    // index mod 8), then the others in turn; a queue seen exhausted is never
+    // asked again, so every workgroup stops after at most n_tiles + 8 claims
+    // in total.
     const uint32_t T = B.n_tiles, home = blockIdx.x % SW_RASTER_QUEUES;
     uint32_t tried = 0;   // thread 0 only: queues found exhausted
     for (;;) {
@@ -986,14 +1106,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             uint32_t tile = 0xFFFFFFFFu;
             while (tried < SW_RASTER_QUEUES) {
                 const uint32_t k = (home + tried) % SW_RASTER_QUEUES;
-                const uint32_t lo = (uint32_t)((uint64_t)T * k / SW_RASTER_QUEUES);
-                const uint32_t hi = (uint32_t)((uint64_t)T * (k + 1) / SW_RASTER_QUEUES);
+                const uint32_t cnt = T > k ? (T - 1u - k) / SW_RASTER_QUEUES + 1u : 0u;   // places of queue k
                 uint32_t *q = &B.ctr[SW_CTR_QUEUE0 + 64 * k];
                 // counters only grow, so a stale read never skips a live queue
-                if (lo + __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < hi) {
+                if (__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < cnt) {
                     const uint32_t i = atomicAdd(q, 1u);
-                    if (lo + i < hi) {
-                        tile = lo + i;
+                    if (i < cnt) {
+                        tile = ro(B.tile_order)[k + SW_RASTER_QUEUES * i];
                         break;
                     }
                 }
@@ -1391,12 +1510,13 @@ __global__ __launch_bounds__(SW_LAYER_THREADS) void k_raster_layers(BatchDev B, 
     uint32_t frags = 0, nans = 0, oob = 0;
     const int lane = __lane_id();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // a drawn triangle (all three vertices marked) whose rectangle holds this tile
     auto covers = [&](uint32_t t) {
         const uint2 r = ro2(J.rect, t);
         return r.x != RECT_EMPTY && tx >= (int)(r.x & 0xFFFF) && tx <= (int)(r.y & 0xFFFF) &&
-               ty >= (int)(r.x >> 16) && ty <= (int)(r.y >> 16);
+               ty >= (int)(r.x >> 16) && ty <= (int)(r.y >> 16) && marked(J, t);
     };
-    const uint32_t nb = B.tile_count[g * SW_TILE_PAD], off = B.tile_offset[g];
+    const uint32_t nb = B.tile_cursor[g * SW_TILE_PAD], off = B.tile_offset[g];
     const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
     // the tile's triangles: bin entries, then the job's big ones whose
     // rectangle holds this tile, while they fit in LDS
@@ -1709,21 +1829,10 @@ hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, uint32_t w0, uint32
     LAUNCH_CHECK(k_vertex, dim3(blocks ? blocks : 1), dim3(256), 0, s, B, w0, n_words, B.jobs);
     return hipSuccess;
 }
-hipError_t launch_mark(const BatchDev &B, uint32_t blocks, hipStream_t s)
+hipError_t launch_front(const BatchDev &B, uint32_t blocks, hipStream_t s)
 {
-    if (blocks)
-        LAUNCH_CHECK(k_mark, dim3(blocks), dim3(256), 0, s, B, B.jobs);
-    return hipSuccess;
-}
-hipError_t launch_bin_count(const BatchDev &B, uint32_t blocks, hipStream_t s)
-{
-    if (blocks)
-        LAUNCH_CHECK(k_bin_count, dim3(blocks), dim3(256), 0, s, B, B.jobs);
-    return hipSuccess;
-}
-hipError_t launch_scan(const BatchDev &B, hipStream_t s)
-{
-    LAUNCH_CHECK(k_scan, dim3(1), dim3(1024), 0, s, B);
+    // at least one block: the last block to finish scans the tile counts
+    LAUNCH_CHECK(k_front, dim3(blocks ? blocks : 1), dim3(256), 0, s, B, B.jobs);
     return hipSuccess;
 }
 hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s)
@@ -1788,8 +1897,7 @@ hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStr
 hipError_t resolve_kernels()
 {
     const void *k[] = {
-        (const void *)k_world, (const void *)k_vertex, (const void *)k_mark, (const void *)k_bin_count,
-        (const void *)k_scan, (const void *)k_bin_fill,
+        (const void *)k_world, (const void *)k_vertex, (const void *)k_front, (const void *)k_bin_fill,
         (const void *)k_raster<8, 8, false>, (const void *)k_raster<16, 16, false>,
         (const void *)k_raster<32, 16, false>, (const void *)k_raster<32, 32, false>,
         (const void *)k_raster<64, 32, false>, (const void *)k_raster<64, 64, false>,

@@ -6,9 +6,8 @@
 namespace sw {
 hipError_t launch_world(const WorldArgs &W, hipStream_t s);
 hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, uint32_t w0, uint32_t n_words, hipStream_t s);
-hipError_t launch_mark(const BatchDev &B, uint32_t blocks, hipStream_t s);
-hipError_t launch_bin_count(const BatchDev &B, uint32_t blocks, hipStream_t s);
-hipError_t launch_scan(const BatchDev &B, hipStream_t s);
+// marks, tile rectangles and counts, and (last block) the count scan
+hipError_t launch_front(const BatchDev &B, uint32_t blocks, hipStream_t s);
 hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s);
 hipError_t launch_raster(const BatchDev &B, int tw, int th, uint32_t max_groups, bool generic, hipStream_t s);
 // A wave of one job with transparency layers (16x16 tiles): the in-order
