@@ -1682,11 +1682,11 @@ __device__ __forceinline__ uint32_t div_count(uint32_t x, uint32_t c)
     return c == 1 ? x : __umulhi(x, k_dof_recip.m[c]);
 }
 
-// LDS bytes of one DoF workgroup for radius R: colour and code of the
-// output tile plus an R halo on every side.
+// LDS bytes of one DoF workgroup for radius R: one tap word per cell of
+// the output tile plus an R halo on every side.
 __host__ __device__ inline size_t dof_lds_bytes(int R)
 {
-    return (size_t)(SW_DOF_TW + 2 * R) * (SW_DOF_TH + 2 * R) * 5 + 4;
+    return (size_t)(SW_DOF_TW + 2 * R) * (SW_DOF_TH + 2 * R) * 4 + 4;
 }
 
 // 256 threads as 64 columns x 4 rows.  Consecutive workgroups run on
@@ -1696,6 +1696,10 @@ __host__ __device__ inline size_t dof_lds_bytes(int R)
 // factor is nonzero (an "unfocused" tap) and its clamped radius.
 // RMAX bounds the wave's halo (the loads below are unrolled for it): the
 // launch picks 8 when no job of the launch has a larger radius.
+// Each LDS cell holds a tap word: an unfocused pixel's b, g, r with 1 in
+// the alpha byte, 0 for a focused pixel or a cell outside the viewport
+// (never an unfocused tap, which is what the reference's clipped loops
+// give), so a tap is one LDS read and its count comes with the colour.
 template <int RMAX>
 __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restrict__ jobs)
 {
@@ -1709,78 +1713,96 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restric
     const int ox = (int)(lb % bx) * SW_DOF_TW, oy = (int)(lb / bx) * SW_DOF_TH;   // viewport-relative
     const int R = J.rmax;
     const int SW = SW_DOF_TW + 2 * R, SH = SW_DOF_TH + 2 * R;
-    uint32_t *s_col = s_dyn;
-    uint8_t *s_code = (uint8_t *)(s_dyn + SW * SH);
+    uint32_t *s_tap = s_dyn;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     // Every load of the halo is issued before the first is used, so the
-    // workgroup waits for memory once rather than once per row.
+    // workgroup waits for memory once rather than once per row.  A cell
+    // outside the viewport loads the tile's first pixel instead (no branch
+    // per cell) and is zeroed after.  Offsets are 32-bit: a surface holds
+    // at most 2^30 words.
     constexpr int ROWS = (SW_DOF_TH + 2 * RMAX + 3) / 4;
     constexpr int COLS = (SW_DOF_TW + 2 * RMAX + 63) / 64;
+    const SW_RO uint32_t *colr = ro(J.out_color);
+    const SW_RO uint8_t *code = ro(J.dofc);
+    const int cbase = (J.V.vy - J.out_oy) * J.out_pitch + (J.V.vx - J.out_ox);   // colour of viewport (0, 0)
+    const int safe_c = cbase + oy * J.out_pitch + ox, safe_k = oy * J.V.vw + ox;
+    int lxs[COLS];
+    bool cok[COLS];
+#pragma unroll
+    for (int m = 0; m < COLS; m++) {
+        const int c = tx + 64 * m;
+        lxs[m] = ox - R + c;
+        cok[m] = c < SW && lxs[m] >= 0 && lxs[m] < J.V.vw;
+    }
     uint32_t cv[ROWS][COLS];
     uint32_t kv[ROWS][COLS];
+    bool okv[ROWS][COLS];
 #pragma unroll
     for (int k = 0; k < ROWS; k++) {
         const int r = ty + 4 * k, ly = oy - R + r;
-        const bool row_ok = r < SH && ly >= 0 && ly < J.V.vh;
-        const uint32_t *crow = J.out_color + (int64_t)(ly + J.V.vy - J.out_oy) * J.out_pitch + (J.V.vx - J.out_ox);
-        const uint8_t *krow = J.dofc + (int64_t)ly * J.V.vw;
+        const bool rok = r < SH && ly >= 0 && ly < J.V.vh;
+        const int crow = cbase + ly * J.out_pitch, krow = ly * J.V.vw;
 #pragma unroll
         for (int m = 0; m < COLS; m++) {
-            const int c = tx + 64 * m, lx = ox - R + c;
-            cv[k][m] = 0u;
-            kv[k][m] = 0u;
-            if (row_ok && c < SW && lx >= 0 && lx < J.V.vw) {
-                cv[k][m] = ro(crow)[lx];   // the raster's output: read-only here
-                kv[k][m] = ro(krow)[lx];
-            }
+            okv[k][m] = rok && cok[m];
+            cv[k][m] = colr[okv[k][m] ? crow + lxs[m] : safe_c];   // the raster's output: read-only here
+            kv[k][m] = code[okv[k][m] ? krow + lxs[m] : safe_k];
         }
     }
-    // Cells outside the viewport keep code 0: a tap there is never an
-    // unfocused pixel, which is what the reference's clipped loops give.
+    // the output pixels' own colours and codes (in the rows just loaded:
+    // cache hits)
+    constexpr int PER = SW_DOF_TH / 4;
+    uint32_t src[PER], own[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int x = ox + tx, y = oy + ty + 4 * j;
+        const bool in = x < J.V.vw && y < J.V.vh;
+        src[j] = colr[in ? cbase + y * J.out_pitch + x : safe_c];
+        own[j] = code[in ? y * J.V.vw + x : safe_k];
+    }
 #pragma unroll
     for (int k = 0; k < ROWS; k++) {
         const int r = ty + 4 * k;
 #pragma unroll
         for (int m = 0; m < COLS; m++) {
             const int c = tx + 64 * m;
-            if (r < SH && c < SW) {
-                s_col[r * SW + c] = cv[k][m];
-                s_code[r * SW + c] = (uint8_t)kv[k][m];
-            }
+            if (r < SH && c < SW)
+                s_tap[r * SW + c] =
+                    okv[k][m] && (kv[k][m] & 1u) ? ((cv[k][m] & 0x00FFFFFFu) | (1u << 24)) : 0u;
         }
     }
     __syncthreads();
-    for (int py = ty; py < SW_DOF_TH; py += 4) {
-        const int px = tx;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int py = ty + 4 * j, px = tx;
         const int x = ox + px, y = oy + py;
         if (x >= J.V.vw || y >= J.V.vh)
             continue;
         const int c = (py + R) * SW + (px + R);
-        const uint32_t src = s_col[c];
-        const int radius = s_code[c] >> 1;
-        uint32_t out = src;
+        const int radius = (int)(own[j] >> 1);
+        uint32_t out = src[j];
         if (radius != 0) {
-            // With radius <= 8 a sum has at most 256 taps of 255, under 2^16, so
-            // blue and red add as the two halves of one word.
+            // With radius <= 8 a sum has at most 256 taps of 255, under 2^16:
+            // blue and red add as the halves of one word, green and the
+            // count as the halves of another.
             constexpr bool PACKED = RMAX <= 8;
-            uint32_t b = 0, g = 0, r = 0, count = 0;
-            auto tap = [&](int k) {   // LDS cell k; adds it when unfocused
-                const uint32_t m = 0u - (uint32_t)(s_code[k] & 1u);
-                const uint32_t qq = s_col[k] & m;
-                count -= m;   // +1 when taken
+            uint32_t br = 0, gc = 0, b = 0, g = 0, r = 0, count = 0;
+            auto tap = [&](int k) {   // LDS cell k
+                const uint32_t w = s_tap[k];
                 if (PACKED) {
-                    b += qq & 0x00FF00FFu;   // blue in the low half, red in the high half
-                    g += (qq >> 8) & 0xFF;
+                    br += w & 0x00FF00FFu;
+                    gc += (w >> 8) & 0x00FF00FFu;
                 } else {
-                    b += qq & 0xFF;
-                    g += (qq >> 8) & 0xFF;
-                    r += (qq >> 16) & 0xFF;
+                    b += w & 0xFF;
+                    g += (w >> 8) & 0xFF;
+                    r += (w >> 16) & 0xFF;
+                    count += w >> 24;
                 }
             };
             if (J.post == 1) {   // box [x-r, x+r) x [y-r, y+r)
-                for (int j = -radius; j < radius; j++)
-                    for (int i = -radius; i < radius; i++)
-                        tap(c + j * SW + i);
+                for (int a = -radius; a < radius; a++)
+                    for (int e = -radius; e < radius; e++)
+                        tap(c + a * SW + e);
             } else {             // 3x3 taps at {-r, 0, r}
                 const int dr = radius * SW;
 #pragma unroll
@@ -1790,8 +1812,10 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restric
                         tap(c + a * dr + e * radius);
             }
             if (PACKED) {
-                r = b >> 16;
-                b &= 0xFFFFu;
+                b = br & 0xFFFFu;
+                r = br >> 16;
+                g = gc & 0xFFFFu;
+                count = gc >> 16;
             }
             if (count)
                 out = (uint32_t)(uint8_t)div_count(b, count) | ((uint32_t)(uint8_t)div_count(g, count) << 8) |
