@@ -493,8 +493,14 @@ def main():
     desc[0] = vp.desc(surf, zt)
     L = lib()
 
+    # The plain path, one frame per render call (swegl_render: the frame's
+    # kernels launched one by one); the same frame as one replayed graph
+    # (swegl_render_graph) is timed beside it.
     def frame(stats=None):
         check(L.swegl_render(ctx.h, ds.h, desc, 1, stats), "swegl_render")
+
+    def frame_graph(stats=None):
+        check(L.swegl_render_graph(ctx.h, ds.h, desc, 1, stats), "swegl_render_graph")
 
     if args.calibrate_copy:
         side = int((args.calibrate_copy * 1024 * 1024 // 4) ** 0.5)
@@ -517,6 +523,14 @@ def main():
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     fps = 1e3 / ms
+    for _ in range(args.warmup):
+        frame_graph()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame_graph()
+    ctx.sync()
+    graph_fps = args.steps / (time.perf_counter() - t0)
 
     # per-frame latency (submit -> complete), frames one at a time
     lat = []
@@ -527,7 +541,7 @@ def main():
         lat.append((time.perf_counter() - a) * 1e3)
     lat = np.array(lat)
 
-    # per-kernel HIP-event timing on the library stream
+    # per-kernel HIP-event timing on the library stream (launched one by one)
     ctx.set_profiling(True)
     for _ in range(10):
         frame()
@@ -580,9 +594,10 @@ def main():
         "data": "synthetic (seeded height field, random texture)",
         "config": {"workload": "heightfield 999,698 triangles, 1920x1080, tex 2048^2, "
                                "sun + 4 point lights, 9-tap DoF r_max 8",
-                   "path": "plain", "tile": "%dx%d" % ctx.get_tile(), "seed": args.seed,
+                   "path": "plain (one render call per frame)", "tile": "%dx%d" % ctx.get_tile(), "seed": args.seed,
                    "tuning": tuning.get("heightfield") and {k: tuning["heightfield"][k] for k in
                                                             ("tile", "bin_limit", "raster_groups")}},
+        "graph_frames_per_s": graph_fps,
         "gfragments_per_s": stats["fragments"] * fps / 1e9,
         "fragments_per_frame": stats["fragments"],
         "shaded_pixels": f_vis,

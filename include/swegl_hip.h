@@ -283,7 +283,9 @@ int  swegl_render_noalloc(swegl_ctx *ctx, swegl_scene *scene,
                           swegl_frame_stats *stats);
 
 /* Plain path as one replayed hipGraph: the first call for a given
- * (scene, viewport set) captures, later calls replay. */
+ * (scene, viewport set) captures, later calls replay.  A viewport set that
+ * combines a depth-of-field post shader with a pixel shader other than
+ * Phong + bilinear renders eagerly instead (same results, no capture). */
 int  swegl_render_graph(swegl_ctx *ctx, swegl_scene *scene,
                         const swegl_viewport_desc *viewports, uint32_t n_viewports,
                         swegl_frame_stats *stats);

@@ -1508,6 +1508,12 @@ static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_
         return rc;
     if (waves.empty())
         return no_waves(stats);
+    // A wave with depth of field and a shader other than Phong + bilinear
+    // runs eagerly: one such graph replay ended in a GPU memory fault whose
+    // cause is not known (the same jobs pass eagerly and in other graphs).
+    for (const Wave &W : waves)
+        if (W.generic && W.dblk)
+            return render_jobs(c, c->lane0, all, stats);
     const std::string key = graph_key(c, waves) + (stats ? "/stats" : "");
     auto it = c->graphs.find(key);
     GraphEntry *G;
