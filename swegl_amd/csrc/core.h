@@ -283,7 +283,8 @@ struct Setup {
 // sub-triangle spans no scanline (y0 == y2).  With `row` given, only the
 // short side of the half that processes that row is set up (the other
 // half's row range is still computed); row_state then works for that row.
-SW_D bool setup_subtri(const SubTri &t, const ViewXf &V, Setup &S, int row = INT32_MIN)
+// fill_triangle_2's y-sort (renderer.cpp:375-387): S.v and S.inverted only.
+SW_D void sort_subtri(const SubTri &t, Setup &S)
 {
     SVert a = t.v[0], b = t.v[1], c = t.v[2];
     S.inverted = !t.front;
@@ -293,7 +294,12 @@ SW_D bool setup_subtri(const SubTri &t, const ViewXf &V, Setup &S, int row = INT
     S.v[0] = a;
     S.v[1] = b;
     S.v[2] = c;
-    const f3 p0 = a.s, p1 = b.s, p2 = c.s;
+}
+
+SW_D bool setup_subtri(const SubTri &t, const ViewXf &V, Setup &S, int row = INT32_MIN)
+{
+    sort_subtri(t, S);
+    const f3 p0 = S.v[0].s, p1 = S.v[1].s, p2 = S.v[2].s;
     S.iy0 = cvtt(ceilf(p0.y));
     S.iy1 = cvtt(ceilf(p1.y));
     S.iy2 = cvtt(ceilf(p2.y));

@@ -702,8 +702,32 @@ struct TileRect { int x0, y0, x1, y1; };   // [x0,x1) x [y0,y1), absolute pixels
 
 // Rows [wy0, wy1) of the tile only, so that one large triangle can be split
 // over several lanes; every row of the tile belongs to exactly one call.
+// Fragment payloads: when a fragment becomes its pixel's current minimum,
+// it records its shading inputs -- its progress u along the span and the
+// row's side progresses pl, pr -- in three 64-bit slots of the pixel, each
+// tagged with its draw sequence and the half and side-order flags (an
+// aligned 64-bit LDS store is single-copy atomic).  The pixel's final
+// winner became the minimum when it ran and wrote all three; a later store
+// by a fragment it beat carries another tag.  So where all three tags name
+// the final winner, the payload is the winner's own, and the shading needs
+// no replay of its setup, rows or span; elsewhere (a race, a sequence of
+// 2^30 or more) the pixel is shaded the long way.
+#define SW_SLOT_SEQ 0x3FFFFFFFu
+__device__ __forceinline__ void put_slots(unsigned long long *slots, int n, int i, uint32_t seq, bool lower, bool lr,
+                                          float u, float pl, float pr)
+{
+    if (seq > SW_SLOT_SEQ)
+        return;
+    const uint64_t tag = (uint64_t)(seq | (lower ? 1u << 30 : 0u) | (lr ? 1u << 31 : 0u)) << 32;
+    __hip_atomic_store(&slots[i], tag | __float_as_uint(u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(&slots[n + i], tag | __float_as_uint(pl), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(&slots[2 * n + i], tag | __float_as_uint(pr), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, int wy1,
-                             unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans)
+                             unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans,
+                             unsigned long long *slots = nullptr, int nslot = 0)
 {
     SubTri st[2];
     int ns = tri_subs<true>(J, t, st);
@@ -734,7 +758,6 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
             }
             Row R;
             row_state(S, y, R);
-            (void)lower;
             for (; y < yend; y++) {
                 Span sp;
                 if (span_of(R, J.V, sp)) {
@@ -742,9 +765,11 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
                     if (xa < xb) {
                         Interp q = sp.q;
                         q.steps(xa - sp.x1);
-                        unsigned long long *row = keys + (y - tr.y0) * TW - tr.x0;
+                        const int rowi = (y - tr.y0) * TW - tr.x0;
+                        unsigned long long *row = keys + rowi;
                         for (int x = xa; x < xb; x++, q.step()) {
-                            float z = q.value(q.u());
+                            const float u = q.u();
+                            float z = q.value(u);
                             if (z < SW_NEAR)
                                 continue;
                             frags++;
@@ -754,7 +779,13 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
                             }
                             if (__float_as_uint(z) >= SW_MAX_Z_BITS)
                                 continue;
-                            atomicMin(&row[x], vis_key(z, seq));
+                            const unsigned long long key = vis_key(z, seq);
+                            if (slots) {
+                                if (key < atomicMin(&row[x], key))
+                                    put_slots(slots, nslot, rowi + x, seq, lower, R.lr, u, sp.pl, sp.pr);
+                            } else {
+                                atomicMin(&row[x], key);
+                            }
                         }
                     }
                 }
@@ -914,10 +945,33 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
     return shade_from<GENERIC>(J, t, S, R, sp, q.u(), oob);
 }
 
+// Shading of the winning sequence seq from its recorded payload (put_slots):
+// the sub-triangle's vertices in fill_triangle_2's order, the half and side
+// order, the side progresses and the span progress u.
+template <bool GENERIC>
+__device__ uint32_t shade_recorded(const JobDev &J, uint32_t seq, uint32_t tag, float u, float pl, float pr,
+                                   uint32_t &oob)
+{
+    const uint32_t t = seq >> 1, s = seq & 1u;
+    SubTri st[2];
+    if ((int)s >= tri_subs<true>(J, t, st))
+        return 0u;   // not reached: the winner came out of this sub-triangle
+    Setup S;
+    sort_subtri(s ? st[1] : st[0], S);
+    Row R;
+    R.lower = (tag >> 30) & 1u;
+    R.lr = (tag >> 31) & 1u;
+    Span sp;
+    sp.pl = pl;
+    sp.pr = pr;
+    return shade_from<GENERIC>(J, t, S, R, sp, u, oob);
+}
+
 template <int TW, int TH, bool GENERIC>
 __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__restrict__ jobs, uint32_t g,
-                                            unsigned long long *keys, uint32_t *s_cnt)
+                                            unsigned long long *keys, uint32_t *s_cnt, unsigned long long *slots)
 {
+    constexpr int NSLOT = TW * TH;
     const int jidx = find_job<JTILE>(B, jobs, g);
     const JobDev &J = jobs[jidx];
     const int lt = (int)(g - J.tile_base);
@@ -958,7 +1012,7 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
                     ty > (int)(r.y >> 16) || !marked(J, t))
                     continue;
             }
-            raster_depth(J, t, tr, tr.y0, tr.y1, keys, TW, frags, nans);
+            raster_depth(J, t, tr, tr.y0, tr.y1, keys, TW, frags, nans, slots, NSLOT);
         }
     } else {
     for (uint32_t base = 0; base < nb + nbig; base += 256) {
@@ -1025,7 +1079,7 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
             const uint32_t pk = s_rows[lo], n = pk >> 16, ch = i - s_first[lo];
             const int r0 = tr.y0 + (int)(pk & 0xFF), rows = (int)((pk >> 8) & 0xFF) - (int)(pk & 0xFF);
             const int ya = r0 + (int)((uint32_t)rows * ch / n), yb = r0 + (int)((uint32_t)rows * (ch + 1) / n);
-            raster_depth(J, s_tri[lo], tr, ya, yb, keys, TW, frags, nans);
+            raster_depth(J, s_tri[lo], tr, ya, yb, keys, TW, frags, nans, slots, NSLOT);
         }
         __syncthreads();   // the LDS tables are rewritten by the next 256 entries
     }
@@ -1044,7 +1098,18 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
         if (k != SW_KEY_EMPTY) {
             seq = (uint32_t)k;
             z = __uint_as_float((uint32_t)(k >> 32));
-            color = shade_pixel<GENERIC>(J, seq, x, y, oob);
+            bool done = false;
+            if (slots) {
+                const unsigned long long a = slots[i], b = slots[NSLOT + i], c = slots[2 * NSLOT + i];
+                const uint32_t tag = (uint32_t)(a >> 32);
+                if (tag == (uint32_t)(b >> 32) && tag == (uint32_t)(c >> 32) && (tag & SW_SLOT_SEQ) == seq) {
+                    color = shade_recorded<GENERIC>(J, seq, tag, __uint_as_float((uint32_t)a),
+                                                    __uint_as_float((uint32_t)b), __uint_as_float((uint32_t)c), oob);
+                    done = true;
+                }
+            }
+            if (!done)
+                color = shade_pixel<GENERIC>(J, seq, x, y, oob);
         }
         J.out_color[(int64_t)(y - J.out_oy) * J.out_pitch + (x - J.out_ox)] = color;
         const int64_t zo = (int64_t)(y - J.V.vy) * J.V.vw + (x - J.V.vx);
@@ -1086,6 +1151,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                                                      const JobDev *__restrict__ jobs)
 {
     __shared__ unsigned long long keys[TW * TH];
+    // fragment payload slots (put_slots) for tiles of at most 32x32 pixels:
+    // 24 KB, which keeps four workgroups on a CU
+    constexpr bool SLOTS = TW * TH <= 1024;
+    __shared__ unsigned long long slots[SLOTS ? 3 * TW * TH : 1];
     __shared__ uint32_t s_cnt[3];
     __shared__ uint32_t s_tile;
     if (threadIdx.x < 3)
@@ -1124,7 +1193,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const uint32_t g = s_tile;
         if (g == 0xFFFFFFFFu)
             break;
-        raster_tile<TW, TH, GENERIC>(B, jobs, g, keys, s_cnt);
+        raster_tile<TW, TH, GENERIC>(B, jobs, g, keys, s_cnt, SLOTS ? slots : nullptr);
         __syncthreads();   // tile done with keys and s_tile before the next claim
     }
     __syncthreads();
