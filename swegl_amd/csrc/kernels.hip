@@ -206,10 +206,10 @@ __device__ SVert clip_vertex(const JobDev &J, uint32_t a, uint32_t b, float cut)
 // BINNED: t came out of the binning, which lists a triangle only if all
 // three of its vertices are marked (quick_extent, and this function with
 // marks checked on the exact path), so the marks need no second look.
+// The same with triangle t's vertex ids already loaded (i0, i1, i2).
 template <bool BINNED = false>
-__device__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
+__device__ int tri_subs_v(const JobDev &J, uint32_t i0, uint32_t i1, uint32_t i2, SubTri *out)
 {
-    uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
     if (!BINNED && !(ro(J.yes)[i0] && ro(J.yes)[i1] && ro(J.yes)[i2]))
         return 0;
     SVert v0 = orig_vertex(J, i0), v1 = orig_vertex(J, i1), v2 = orig_vertex(J, i2);
@@ -255,6 +255,11 @@ __device__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
         return 2;
     }
     return 0;   // NaN depth: no branch of the reference is taken
+}
+template <bool BINNED = false>
+__device__ __forceinline__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
+{
+    return tri_subs_v<BINNED>(J, ro(J.tri)[3 * t], ro(J.tri)[3 * t + 1], ro(J.tri)[3 * t + 2], out);
 }
 
 // Rows a sub-triangle's halves process, and a conservative column range.
@@ -712,13 +717,28 @@ struct TileRect { int x0, y0, x1, y1; };   // [x0,x1) x [y0,y1), absolute pixels
 // the final winner, the payload is the winner's own, and the shading needs
 // no replay of its setup, rows or span; elsewhere (a race, a sequence of
 // 2^30 or more) the pixel is shaded the long way.
-#define SW_SLOT_SEQ 0x3FFFFFFFu
-__device__ __forceinline__ void put_slots(unsigned long long *slots, int n, int i, uint32_t seq, bool lower, bool lr,
-                                          float u, float pl, float pr)
+// Tag: the sequence (25 bits), then the sub-triangle's vertex order in
+// fill_triangle_2 as (index of S.v[0], of S.v[1]) among the triangle's
+// three vertices, 3 * p0 + p1 (15: a clip vertex is among them), whether it
+// is inverted, the half and the side order.
+#define SW_SLOT_SEQ 0x01FFFFFFu
+#define SW_SLOT_NOPERM 15u
+__device__ __forceinline__ uint32_t slot_code(const Setup &S, uint32_t i0, uint32_t i1, uint32_t i2)
+{
+    if (S.v[0].clip | S.v[1].clip | S.v[2].clip)
+        return SW_SLOT_NOPERM | (S.inverted ? 16u : 0u);
+    const uint32_t p0 = S.v[0].a == i0 ? 0u : S.v[0].a == i1 ? 1u : 2u;
+    const uint32_t p1 = S.v[1].a == i0 ? 0u : S.v[1].a == i1 ? 1u : 2u;
+    if (p0 == p1)   // repeated vertex ids: no single order to record
+        return SW_SLOT_NOPERM | (S.inverted ? 16u : 0u);
+    return 3u * p0 + p1 + (S.inverted ? 16u : 0u);
+}
+__device__ __forceinline__ void put_slots(unsigned long long *slots, int n, int i, uint32_t seq, uint32_t code,
+                                          bool lower, bool lr, float u, float pl, float pr)
 {
     if (seq > SW_SLOT_SEQ)
         return;
-    const uint64_t tag = (uint64_t)(seq | (lower ? 1u << 30 : 0u) | (lr ? 1u << 31 : 0u)) << 32;
+    const uint64_t tag = (uint64_t)(seq | code << 25 | (lower ? 1u << 30 : 0u) | (lr ? 1u << 31 : 0u)) << 32;
     __hip_atomic_store(&slots[i], tag | __float_as_uint(u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __hip_atomic_store(&slots[n + i], tag | __float_as_uint(pl), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __hip_atomic_store(&slots[2 * n + i], tag | __float_as_uint(pr), __ATOMIC_RELAXED,
@@ -730,13 +750,15 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
                              unsigned long long *slots = nullptr, int nslot = 0)
 {
     SubTri st[2];
-    int ns = tri_subs<true>(J, t, st);
+    const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
+    int ns = tri_subs_v<true>(J, i0, i1, i2, st);
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         Setup S;
         if (s >= ns || !setup_subtri(st[s], J.V, S))
             continue;
         const uint32_t seq = 2u * t + (uint32_t)s;
+        const uint32_t code = slots ? slot_code(S, i0, i1, i2) : 0u;
         int y = wy0;
         while (y < wy1) {
             bool lower;
@@ -782,7 +804,7 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
                             const unsigned long long key = vis_key(z, seq);
                             if (slots) {
                                 if (key < atomicMin(&row[x], key))
-                                    put_slots(slots, nslot, rowi + x, seq, lower, R.lr, u, sp.pl, sp.pr);
+                                    put_slots(slots, nslot, rowi + x, seq, code, lower, R.lr, u, sp.pl, sp.pr);
                             } else {
                                 atomicMin(&row[x], key);
                             }
@@ -952,12 +974,26 @@ template <bool GENERIC>
 __device__ uint32_t shade_recorded(const JobDev &J, uint32_t seq, uint32_t tag, float u, float pl, float pr,
                                    uint32_t &oob)
 {
-    const uint32_t t = seq >> 1, s = seq & 1u;
-    SubTri st[2];
-    if ((int)s >= tri_subs<true>(J, t, st))
-        return 0u;   // not reached: the winner came out of this sub-triangle
+    const uint32_t t = seq >> 1, s = seq & 1u, code = (tag >> 25) & 31u, perm = code & 15u;
     Setup S;
-    sort_subtri(s ? st[1] : st[0], S);
+    if (perm < 9u) {
+        // fill_triangle_2's vertex order, recorded: no z- or y-sort to redo
+        const uint32_t id[3] = {ro(J.tri)[3 * t], ro(J.tri)[3 * t + 1], ro(J.tri)[3 * t + 2]};
+        const uint32_t p0 = perm / 3u, p1 = perm % 3u, p2 = 3u - p0 - p1;
+        const uint32_t ids[3] = {id[p0], id[p1], id[p2]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            S.v[k].a = S.v[k].b = ids[k];
+            S.v[k].cut = 0.0f;
+            S.v[k].clip = 0;
+        }
+        S.inverted = (code & 16u) != 0;
+    } else {
+        SubTri st[2];
+        if ((int)s >= tri_subs<true>(J, t, st))
+            return 0u;   // not reached: the winner came out of this sub-triangle
+        sort_subtri(s ? st[1] : st[0], S);
+    }
     Row R;
     R.lower = (tag >> 30) & 1u;
     R.lr = (tag >> 31) & 1u;
