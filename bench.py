@@ -346,7 +346,7 @@ def bench_heavy(ctx, A, check, L, count, seed, iters, plain_mode="graph"):
     return out
 
 
-def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fracs):
+def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fracs, plain_mode="graph"):
     """Workload 'served': n_req requests, 95% drawn from the batch scenes and
     5% of the 1M-triangle 1080p shape, each with a seeded camera pose,
     admitted against a 200 GB budget.  Reports the plain path (each request
@@ -391,25 +391,27 @@ def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fr
     # plain: every request alone, in arrival order (graphs captured first);
     # the warm-up render of each scene's first request gives its shaded
     # pixels, which stand for every request of that scene in the texel term
+    plain_fn = L.swegl_render_graph if plain_mode == "graph" else L.swegl_render
     fv = {}
     for k in range(len(allp)):
         i = int(np.argmax(which == k)) if (which == k).any() else None
         if i is not None:
-            check(L.swegl_render_graph(ctx.h, ds[k].h, C.pointer(descs[i]), 1, None), "warm")
+            check(plain_fn(ctx.h, ds[k].h, C.pointer(descs[i]), 1, None), "warm")
             ctx.sync()
             fv[k] = shaded_pixels(target(allp[k][1][0], i)[1])
     ctx.sync()
     req_bytes = sum(job_bytes(allp[int(k)][0], allp[int(k)][1][:1], [fv[int(k)]]) for k in which)
     a = time.perf_counter()
     for i in range(n_req):
-        check(L.swegl_render_graph(ctx.h, ds[int(which[i])].h, C.pointer(descs[i]), 1, None), "plain")
+        check(plain_fn(ctx.h, ds[int(which[i])].h, C.pointer(descs[i]), 1, None), "plain")
     ctx.sync()
     plain_fps = n_req / (time.perf_counter() - a)
 
     plain_s = n_req / plain_fps
     out = {"requests": n_req, "large_requests": n_big, "lanes": lanes, "max_batch": max_batch,
            "hbm_budget_bytes": 200 * 10**9, "algorithmic_bytes": req_bytes,
-           "plain_alone": {"frames_per_s": plain_fps, "roofline": roofline_of(req_bytes, plain_s)}}
+           "plain_alone": {"frames_per_s": plain_fps, "roofline": roofline_of(req_bytes, plain_s),
+                           "path": "graph replay per request" if plain_mode == "graph" else "swegl_render per request"}}
     srv = Server(ctx, 200 * 10**9, max_batch=max_batch, lanes=lanes)
     # saturated: everything queued at t=0
     lat = srv.run_trace(jobs, np.zeros(n_req))
@@ -624,7 +626,7 @@ def main():
                                           args.plain)
         if args.requests:
             out["served"] = bench_served(ctx, A, check, L, pairs, (scene, vps), args.requests,
-                                         args.seed + 2, args.lanes, args.max_batch, (0.3, 0.6, 0.9))
+                                         args.seed + 2, args.lanes, args.max_batch, (0.3, 0.6, 0.9), args.plain)
     if args.heavy:
         if "heavy" in tuning:
             ctx.apply_tuning(tuning["heavy"])
