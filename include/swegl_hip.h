@@ -249,7 +249,10 @@ int  swegl_scene_bytes(swegl_ctx *ctx, swegl_scene *scene, uint64_t *bytes);
 /* ---- render: swegl::render(scene, vp...) ------------------------------- */
 /* Replaces renderer.hpp:27-34: world transform once, then each viewport in
  * order (renderer.cpp:77-235), including clear (viewport.cpp:88-113) and the
- * post shader.  stats may be NULL. */
+ * post shader.  stats may be NULL.  Asynchronous: results are complete after
+ * swegl_ctx_synchronize or a target read; renders queued back to back
+ * overlap (one render's binning runs while the previous one's raster
+ * finishes) and write their targets in call order. */
 int  swegl_render(swegl_ctx *ctx, swegl_scene *scene,
                   const swegl_viewport_desc *viewports, uint32_t n_viewports,
                   swegl_frame_stats *stats);

@@ -85,8 +85,12 @@ struct swegl_target {
     DevBuf buf;
 };
 
+// Per-viewport scratch of the context lane.  The front end's buffers come in
+// two sets (the eager path's parity, see swegl_ctx), so that one render's
+// front end can run while the previous render's raster still reads its own.
 struct ViewScratch {
     DevBuf ndc, yes, rect, big, dof, tflat;
+    DevBuf ndc1, yes1, rect1, big1;
 };
 
 struct swegl_scene {
@@ -194,7 +198,7 @@ struct GraphEntry {
 // one region whatever scenes pass through it.
 struct Lane {
     hipStream_t stream = nullptr;
-    Resources res;
+    Resources res, res1;   // res1: the context lane's second set (parity 1)
     DevBuf region;
     StageSlot stage[4];
     int stage_next = 0;
@@ -225,6 +229,21 @@ struct swegl_ctx {
     size_t graph_bytes = 0;                // device + pinned bytes of the cached graphs
     std::vector<Lane *> server_lanes;      // lanes of this context's servers (scene scratch)
     char *scratch = nullptr;               // this device's fixed scratch array (allocation-free path)
+    // The eager path's two-stream pipeline.  A render with one launch wave
+    // runs its front end (job table upload, k_vertex, k_front, k_bin_fill)
+    // on fstream and its raster and depth of field on the context stream,
+    // which waits for that front end.  Renders alternate between two sets
+    // of front-end buffers (parity), so the next render's front end only
+    // waits for the raster that last used its set, two renders back, and
+    // runs while the previous render's raster and depth of field finish.
+    // Any other work on the context stream (uploads, pose updates, target
+    // writes, other render paths) bumps op_seq; the next pipelined render
+    // then starts its front end after everything on the context stream.
+    hipStream_t fstream = nullptr;
+    hipEvent_t f_done[2] = {nullptr, nullptr}, r_done[2] = {nullptr, nullptr}, tail = nullptr;
+    bool r_recorded[2] = {false, false};
+    int parity = 0;
+    uint64_t op_seq = 1, pipe_seq = 0;   // equal: only pipelined renders since the last one
     std::vector<char> noalloc_tables;      // host job tables of the allocation-free path
     hipEvent_t noalloc_done = nullptr;     // their last upload
     bool noalloc_used = false;
@@ -299,17 +318,19 @@ static int stage_acquire(Lane &L, size_t bytes, StageSlot **out)
     return SWEGL_OK;
 }
 
-static int upload(Lane &L, void *dst, const void *src, size_t bytes)
+static int upload(Lane &L, void *dst, const void *src, size_t bytes, hipStream_t st = nullptr)
 {
     if (!bytes)
         return SWEGL_OK;
+    if (!st)
+        st = L.stream;
     StageSlot *s;
     int rc = stage_acquire(L, bytes, &s);
     if (rc)
         return rc;
     memcpy(s->host, src, bytes);
-    HIPCHK(hipMemcpyAsync(dst, s->host, bytes, hipMemcpyHostToDevice, L.stream));
-    HIPCHK(hipEventRecord(s->done, L.stream));
+    HIPCHK(hipMemcpyAsync(dst, s->host, bytes, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(s->done, st));
     s->used = true;
     return SWEGL_OK;
 }
@@ -408,6 +429,12 @@ extern "C" int swegl_ctx_create(int device, swegl_ctx **out)
     if (src)
         return src;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->fstream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; k++) {
+        HIPCHK(hipEventCreateWithFlags(&c->f_done[k], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->r_done[k], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&c->tail, hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void **)&c->h_stats, SW_CTR_CONTROL * sizeof(uint32_t), hipHostMallocDefault));
     HIPCHK(hipEventCreateWithFlags(&c->noalloc_done, hipEventDisableTiming));
     c->lane0.stream = c->stream;
@@ -423,6 +450,8 @@ extern "C" int swegl_ctx_destroy(swegl_ctx *c)
     if (!c)
         return SWEGL_OK;
     (void)hipSetDevice(c->device);
+    if (c->fstream)
+        (void)hipStreamSynchronize(c->fstream);
     (void)hipStreamSynchronize(c->stream);
     drain_profile(c);
     for (hipEvent_t e : c->event_pool)
@@ -434,6 +463,11 @@ extern "C" int swegl_ctx_destroy(swegl_ctx *c)
     if (c->h_stage)
         (void)hipHostFree(c->h_stage);
     c->graphs.clear();
+    for (hipEvent_t e : {c->f_done[0], c->f_done[1], c->r_done[0], c->r_done[1], c->tail})
+        if (e)
+            (void)hipEventDestroy(e);
+    if (c->fstream)
+        (void)hipStreamDestroy(c->fstream);
     (void)hipStreamDestroy(c->stream);
     c->lane0.stream = nullptr;   // destroyed above: the lane must not wait on it
     delete c;
@@ -444,6 +478,7 @@ extern "C" int swegl_ctx_synchronize(swegl_ctx *c)
 {
     if (!c)
         return fail(SWEGL_EINVAL, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->fstream));
     HIPCHK(hipStreamSynchronize(c->stream));
     drain_profile(c);
     return SWEGL_OK;
@@ -701,6 +736,7 @@ static int node_matrices(swegl_scene *s, std::vector<NodeDev> &out)
 
 static int scene_pose(swegl_ctx *c, swegl_scene *s, const swegl_scene_desc *d)
 {
+    c->op_seq++;   // world, normals, nodes and lights change on the context stream
     s->h_nodes.assign(d->nodes, d->nodes + d->n_nodes);
     s->roots.assign(d->root_nodes, d->root_nodes + d->n_root_nodes);
     s->children.assign(d->child_indices, d->child_indices + d->n_child_indices);
@@ -1049,7 +1085,7 @@ static size_t job_scratch_sizes(const swegl_scene *s, const swegl_viewport_desc 
     return sz[0] + sz[1] + sz[2] + sz[3] + sz[4] + sz[5];
 }
 static int job_scratch(Lane &L, Arena *arena, swegl_scene *s, int use, const swegl_viewport_desc &v,
-                       JobScratch &out)
+                       JobScratch &out, int parity)
 {
     size_t sz[6];
     job_scratch_sizes(s, v, sz);
@@ -1061,7 +1097,8 @@ static int job_scratch(Lane &L, Arena *arena, swegl_scene *s, int use, const swe
                                           "the %zu bytes of the fixed scratch array", arena->cap);
     } else {
         ViewScratch &vs = *L.scratch(s, use);
-        DevBuf *b[6] = {&vs.ndc, &vs.yes, &vs.rect, &vs.big, &vs.dof, &vs.tflat};
+        DevBuf *b[6] = {parity ? &vs.ndc1 : &vs.ndc, parity ? &vs.yes1 : &vs.yes, parity ? &vs.rect1 : &vs.rect,
+                        parity ? &vs.big1 : &vs.big, &vs.dof, &vs.tflat};
         for (int k = 0; k < 6; k++)
             if (sz[k]) {
                 int rc = b[k]->ensure(sz[k]);
@@ -1079,7 +1116,7 @@ static int job_scratch(Lane &L, Arena *arena, swegl_scene *s, int use, const swe
     return SWEGL_OK;
 }
 
-static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wave &W, Arena *arena)
+static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wave &W, Arena *arena, int parity)
 {
     const uint32_t nj = (uint32_t)jobs.size();
     W.jd.assign(nj, JobDev{});
@@ -1090,7 +1127,7 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         swegl_scene *s = jobs[j].scene;
         const swegl_viewport_desc &v = *jobs[j].vp;
         JobScratch vs;
-        int rc = job_scratch(L, arena, s, scene_uses[s]++, v, vs);
+        int rc = job_scratch(L, arena, s, scene_uses[s]++, v, vs, parity);
         if (rc)
             return rc;
         const bool dof = v.post_shader != SWEGL_POST_NULL;
@@ -1250,8 +1287,11 @@ static int carve_resources(Arena &A, std::vector<Wave> &waves, ResPtr &out)
 
 // Enqueue the kernels of one wave (no allocation: safe under graph capture).
 // `untimed`: no profiling events (capture, and the allocation-free path).
+// The front-end stages go on `fs` (the lane's stream when null); with
+// another stream, `fdone` is recorded there after them and the lane's
+// stream waits for it before the raster.
 static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bool first_wave, bool untimed,
-                       bool want_stats)
+                       bool want_stats, hipStream_t fs = nullptr, hipEvent_t fdone = nullptr)
 {
     const bool capturing = untimed;
     BatchDev B;
@@ -1272,28 +1312,34 @@ static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bo
     const uint32_t zero0 = first_wave ? 0 : SW_CTR_CONTROL;
     const uint32_t groups = (uint32_t)std::max(1, c->n_cus) * c->groups_per_cu;
     hipStream_t st = L.stream;
-#define STAGE(name, call)                                                              \
+#define STAGE(name, strm, call)                                                        \
     do {                                                                               \
-        hipError_t e_ = timed(c, st, name, capturing, [&] { return call; });            \
+        hipError_t e_ = timed(c, strm, name, capturing, [&] { return call; });          \
         if (e_ != hipSuccess)                                                          \
             return fail(SWEGL_EIO, "%s launch: %s", name, hipGetErrorString(e_));       \
     } while (0)
-    STAGE("vertex", launch_vertex(B, W.vblk, zero0, W.n_ctr - zero0, st));
-    STAGE("front", launch_front(B, W.tblk, st));
-    STAGE("bin_fill", launch_bin_fill(B, W.tblk, st));
+    if (!fs)
+        fs = st;
+    STAGE("vertex", fs, launch_vertex(B, W.vblk, zero0, W.n_ctr - zero0, fs));
+    STAGE("front", fs, launch_front(B, W.tblk, fs));
+    STAGE("bin_fill", fs, launch_bin_fill(B, W.tblk, fs));
+    if (fs != st) {
+        HIPCHK(hipEventRecord(fdone, fs));
+        HIPCHK(hipStreamWaitEvent(st, fdone, 0));
+    }
     if (W.layers) {
         const uint32_t pixels = (uint32_t)(W.jd[0].V.vw * W.jd[0].V.vh);
-        STAGE("raster", launch_layers(B, W.generic, pixels, st));
+        STAGE("raster", st, launch_layers(B, W.generic, pixels, st));
     } else {
-        STAGE("raster", launch_raster(B, W.tw, W.th, groups, W.generic, st));
-        STAGE("dof", launch_dof(B, W.dblk, W.rmax, st));
+        STAGE("raster", st, launch_raster(B, W.tw, W.th, groups, W.generic, st));
+        STAGE("dof", st, launch_dof(B, W.dblk, W.rmax, st));
     }
 #undef STAGE
     return SWEGL_OK;
 }
 
 static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::vector<Wave> &waves,
-                   Arena *arena = nullptr)
+                   Arena *arena = nullptr, int parity = 0)
 {
     HIPCHK(hipSetDevice(c->device));
     for (const JobIn &j : all) {
@@ -1328,7 +1374,7 @@ static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::ve
         if (groups[w].empty())
             continue;
         waves.emplace_back();
-        int rc = build_wave(c, L, groups[w], waves.back(), arena);
+        int rc = build_wave(c, L, groups[w], waves.back(), arena, parity);
         if (rc)
             return rc;
     }
@@ -1400,13 +1446,46 @@ static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swe
         if (waves.empty())
             return no_waves(stats);
     } else {
-        if ((rc = prepare(c, L, all, waves)))
+        // the context lane: the front-end buffers of this render's parity
+        const int p = c->parity;
+        c->parity ^= 1;
+        if ((rc = prepare(c, L, all, waves, nullptr, p)))
             return rc;
         if (waves.empty())
             return no_waves(stats);
-        if ((rc = size_resources(L.res, waves)))
+        Resources &RS = p ? L.res1 : L.res;
+        if ((rc = size_resources(RS, waves)))
             return rc;
-        R = res_ptr(L.res);
+        R = res_ptr(RS);
+        if (waves.size() == 1 && !waves[0].layers && !c->prof) {
+            // Pipelined (not while kernels are timed, so that each stage's
+            // events time that stage alone): the front end on fstream, after
+            // the raster that last used this parity's buffers, or after
+            // everything on the context stream when other work was queued
+            // there since the last pipelined render
+            hipStream_t fs = c->fstream;
+            if (c->pipe_seq != c->op_seq) {
+                HIPCHK(hipEventRecord(c->tail, c->stream));
+                HIPCHK(hipStreamWaitEvent(fs, c->tail, 0));
+            } else if (c->r_recorded[p]) {
+                HIPCHK(hipStreamWaitEvent(fs, c->r_done[p], 0));
+            }
+            c->r_recorded[p] = false;
+            c->pipe_seq = c->op_seq - 1;   // until this render is fully queued
+            if ((rc = upload(L, R.jobs + waves[0].job_offset, waves[0].jd.data(),
+                             waves[0].jd.size() * sizeof(JobDev), fs)) ||
+                (rc = launch_wave(c, L, R, waves[0], true, false, stats != nullptr, fs, c->f_done[p]))) {
+                // whatever reached fstream is waited for by the context stream
+                (void)hipEventRecord(c->f_done[p], fs);
+                (void)hipStreamWaitEvent(c->stream, c->f_done[p], 0);
+                return rc;
+            }
+            HIPCHK(hipEventRecord(c->r_done[p], c->stream));
+            c->r_recorded[p] = true;
+            c->pipe_seq = c->op_seq;
+            return stats ? read_stats(c, L, R.ctr, stats) : SWEGL_OK;
+        }
+        c->op_seq++;   // serial: the next pipelined render waits for it
     }
     for (size_t w = 0; w < waves.size(); w++) {
         if ((rc = upload(L, R.jobs + waves[w].job_offset, waves[w].jd.data(),
@@ -1424,6 +1503,7 @@ static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swe
 // do not fit are refused before anything is enqueued.
 static int render_jobs_noalloc(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
 {
+    c->op_seq++;
     if (!c->scratch)
         return fail(SWEGL_EIO, "no scratch array address for device %d", c->device);
     Arena A;
@@ -1502,6 +1582,7 @@ static int evict_graphs(swegl_ctx *c)
 
 static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
 {
+    c->op_seq++;   // the per-viewport scratch of parity 0, on the context stream
     std::vector<Wave> waves;
     int rc = prepare(c, c->lane0, all, waves);
     if (rc)
@@ -1606,6 +1687,7 @@ extern "C" int swegl_debug_vertex_state(swegl_ctx *c, swegl_scene *s, const sweg
 {
     if (!c || !s || !vp || (s->n_vertices && (!v_world || !normal_world || !v_viewport || !yes)))
         return fail(SWEGL_EINVAL, "null argument");
+    const int parity = c->parity;   // the set of front-end buffers the render below uses
     int rc = swegl_render(c, s, vp, 1, nullptr);
     if (rc)
         return rc;
@@ -1620,8 +1702,8 @@ extern "C" int swegl_debug_vertex_state(swegl_ctx *c, swegl_scene *s, const sweg
     std::vector<uint8_t> y(V);
     HIPCHK(hipMemcpy(w.data(), s->world.p, V * sizeof(float4), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(n.data(), s->nworld.p, V * sizeof(float4), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(f.data(), vs.ndc.p, V * sizeof(float4), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(y.data(), vs.yes.p, V, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(f.data(), (parity ? vs.ndc1 : vs.ndc).p, V * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(y.data(), (parity ? vs.yes1 : vs.yes).p, V, hipMemcpyDeviceToHost));
     float m00, m03, m11, m13;
     hm::viewport_matrix(vp->x, vp->y, vp->w, vp->h, &m00, &m03, &m11, &m13);
     for (size_t i = 0; i < V; i++) {
