@@ -745,6 +745,22 @@ __device__ __forceinline__ void put_slots(unsigned long long *slots, int n, int 
                        __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// c ? a : b, field by field: a select of whole structs would take their
+// addresses and put the setup in private memory.
+__device__ __forceinline__ Side pick(bool c, const Side &a, const Side &b)
+{
+    Side r;
+    r.ratio = c ? a.ratio : b.ratio;
+    r.x = c ? a.x : b.x;
+    r.it.ta = c ? a.it.ta : b.it.ta;
+    r.it.ts = c ? a.it.ts : b.it.ts;
+    r.it.ba = c ? a.it.ba : b.it.ba;
+    r.it.bs = c ? a.it.bs : b.it.bs;
+    r.it.v0 = c ? a.it.v0 : b.it.v0;
+    r.it.v1 = c ? a.it.v1 : b.it.v1;
+    return r;
+}
+
 __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, int wy1,
                              unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans,
                              unsigned long long *slots = nullptr, int nslot = 0)
@@ -759,61 +775,71 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
             continue;
         const uint32_t seq = 2u * t + (uint32_t)s;
         const uint32_t code = slots ? slot_code(S, i0, i1, i2) : 0u;
-        int y = wy0;
-        while (y < wy1) {
-            bool lower;
-            int yend;
-            if (S.has_up && y >= S.up_y0 && y < S.up_y1) {
-                lower = false;
-                yend = min(S.up_y1, wy1);
-            } else if (S.has_lo && y >= S.lo_y0 && y < S.lo_y1) {
+        // The drawn rows within [wy0, wy1), walked in one loop: the upper
+        // half's rows [up_y0, up_y1) and the lower half's [lo_y0, lo_y1) are
+        // adjacent when both exist (up_y1 == lo_y0 == iy1).  The long side
+        // has taken y - ysL steps at row y in either half (row_state's two
+        // counts agree), a short side y minus its half's first row; at the
+        // switch the lower short side starts afresh from its setup state.
+        // So every lane runs one loop over its rows, the same adds in the
+        // same order as the per-half replay.
+        const bool up = S.has_up && S.up_y1 > S.up_y0, lo = S.has_lo && S.lo_y1 > S.lo_y0;
+        if (!up && !lo)
+            continue;
+        const int ya = max(wy0, up ? S.up_y0 : S.lo_y0), yb = min(wy1, lo ? S.lo_y1 : S.up_y1);
+        if (ya >= yb)
+            continue;
+        bool lower = !(up && ya < S.up_y1);
+        Row R;
+        {
+            Side lng = S.L, sh = pick(lower, S.Sl, S.Su);
+            lng.steps(ya - S.ysL);
+            sh.steps(ya - (lower ? S.lo_y0 : S.up_y0));
+            R.lr = lower ? S.lr_lo : S.lr_up;
+            R.left = pick(R.lr, sh, lng);
+            R.right = pick(R.lr, lng, sh);
+        }
+        for (int y = ya; y < yb; y++) {
+            if (!lower && y == S.up_y1) {   // into the lower half
+                const Side lng = pick(R.lr, R.right, R.left);
                 lower = true;
-                yend = min(S.lo_y1, wy1);
-            } else {
-                int nxt = wy1;
-                if (S.has_up && S.up_y0 > y && S.up_y1 > S.up_y0)
-                    nxt = min(nxt, S.up_y0);
-                if (S.has_lo && S.lo_y0 > y && S.lo_y1 > S.lo_y0)
-                    nxt = min(nxt, S.lo_y0);
-                y = nxt;
-                continue;
+                R.lr = S.lr_lo;
+                R.left = pick(R.lr, S.Sl, lng);
+                R.right = pick(R.lr, lng, S.Sl);
             }
-            Row R;
-            row_state(S, y, R);
-            for (; y < yend; y++) {
-                Span sp;
-                if (span_of(R, J.V, sp)) {
-                    int xa = max(sp.x1, tr.x0), xb = min(sp.x2, tr.x1);
-                    if (xa < xb) {
-                        Interp q = sp.q;
-                        q.steps(xa - sp.x1);
-                        const int rowi = (y - tr.y0) * TW - tr.x0;
-                        unsigned long long *row = keys + rowi;
-                        for (int x = xa; x < xb; x++, q.step()) {
-                            const float u = q.u();
-                            float z = q.value(u);
-                            if (z < SW_NEAR)
-                                continue;
-                            frags++;
-                            if (z != z) {
-                                nans++;
-                                continue;
-                            }
-                            if (__float_as_uint(z) >= SW_MAX_Z_BITS)
-                                continue;
-                            const unsigned long long key = vis_key(z, seq);
-                            if (slots) {
-                                if (key < atomicMin(&row[x], key))
-                                    put_slots(slots, nslot, rowi + x, seq, code, lower, R.lr, u, sp.pl, sp.pr);
-                            } else {
-                                atomicMin(&row[x], key);
-                            }
+            R.lower = lower;
+            Span sp;
+            if (span_of(R, J.V, sp)) {
+                int xa = max(sp.x1, tr.x0), xb = min(sp.x2, tr.x1);
+                if (xa < xb) {
+                    Interp q = sp.q;
+                    q.steps(xa - sp.x1);
+                    const int rowi = (y - tr.y0) * TW - tr.x0;
+                    unsigned long long *row = keys + rowi;
+                    for (int x = xa; x < xb; x++, q.step()) {
+                        const float u = q.u();
+                        float z = q.value(u);
+                        if (z < SW_NEAR)
+                            continue;
+                        frags++;
+                        if (z != z) {
+                            nans++;
+                            continue;
+                        }
+                        if (__float_as_uint(z) >= SW_MAX_Z_BITS)
+                            continue;
+                        const unsigned long long key = vis_key(z, seq);
+                        if (slots) {
+                            if (key < atomicMin(&row[x], key))
+                                put_slots(slots, nslot, rowi + x, seq, code, lower, R.lr, u, sp.pl, sp.pr);
+                        } else {
+                            atomicMin(&row[x], key);
                         }
                     }
                 }
-                R.left.steps(1);
-                R.right.steps(1);
             }
+            R.left.steps(1);
+            R.right.steps(1);
         }
     }
 }
