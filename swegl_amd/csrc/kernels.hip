@@ -761,6 +761,48 @@ __device__ __forceinline__ Side pick(bool c, const Side &a, const Side &b)
     return r;
 }
 
+// Calls row(y, R) for every row y in [wy0, wy1) that fill_triangle_2 draws
+// for S (renderer.cpp:462-558), R holding the row's sides, half and side
+// order.  The upper half's rows [up_y0, up_y1) and the lower half's
+// [lo_y0, lo_y1) are adjacent when both exist (up_y1 == lo_y0 == iy1), so
+// one loop walks them: the long side has taken y - ysL steps at row y in
+// either half (row_state's two counts agree), a short side y minus its
+// half's first row, and at the switch the lower short side starts from its
+// setup state -- the same adds in the same order as a per-half replay,
+// with every lane of a wave in one loop over its own rows.
+template <typename RowF>
+__device__ __forceinline__ void walk_rows(const Setup &S, int wy0, int wy1, RowF row)
+{
+    const bool up = S.has_up && S.up_y1 > S.up_y0, lo = S.has_lo && S.lo_y1 > S.lo_y0;
+    if (!up && !lo)
+        return;
+    const int ya = max(wy0, up ? S.up_y0 : S.lo_y0), yb = min(wy1, lo ? S.lo_y1 : S.up_y1);
+    if (ya >= yb)
+        return;
+    Row R;
+    R.lower = !(up && ya < S.up_y1);
+    {
+        Side lng = S.L, sh = pick(R.lower, S.Sl, S.Su);
+        lng.steps(ya - S.ysL);
+        sh.steps(ya - (R.lower ? S.lo_y0 : S.up_y0));
+        R.lr = R.lower ? S.lr_lo : S.lr_up;
+        R.left = pick(R.lr, sh, lng);
+        R.right = pick(R.lr, lng, sh);
+    }
+    for (int y = ya; y < yb; y++) {
+        if (!R.lower && y == S.up_y1) {   // into the lower half
+            const Side lng = pick(R.lr, R.right, R.left);
+            R.lower = true;
+            R.lr = S.lr_lo;
+            R.left = pick(R.lr, S.Sl, lng);
+            R.right = pick(R.lr, lng, S.Sl);
+        }
+        row(y, R);
+        R.left.steps(1);
+        R.right.steps(1);
+    }
+}
+
 __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, int wy1,
                              unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans,
                              unsigned long long *slots = nullptr, int nslot = 0)
@@ -775,72 +817,38 @@ __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, 
             continue;
         const uint32_t seq = 2u * t + (uint32_t)s;
         const uint32_t code = slots ? slot_code(S, i0, i1, i2) : 0u;
-        // The drawn rows within [wy0, wy1), walked in one loop: the upper
-        // half's rows [up_y0, up_y1) and the lower half's [lo_y0, lo_y1) are
-        // adjacent when both exist (up_y1 == lo_y0 == iy1).  The long side
-        // has taken y - ysL steps at row y in either half (row_state's two
-        // counts agree), a short side y minus its half's first row; at the
-        // switch the lower short side starts afresh from its setup state.
-        // So every lane runs one loop over its rows, the same adds in the
-        // same order as the per-half replay.
-        const bool up = S.has_up && S.up_y1 > S.up_y0, lo = S.has_lo && S.lo_y1 > S.lo_y0;
-        if (!up && !lo)
-            continue;
-        const int ya = max(wy0, up ? S.up_y0 : S.lo_y0), yb = min(wy1, lo ? S.lo_y1 : S.up_y1);
-        if (ya >= yb)
-            continue;
-        bool lower = !(up && ya < S.up_y1);
-        Row R;
-        {
-            Side lng = S.L, sh = pick(lower, S.Sl, S.Su);
-            lng.steps(ya - S.ysL);
-            sh.steps(ya - (lower ? S.lo_y0 : S.up_y0));
-            R.lr = lower ? S.lr_lo : S.lr_up;
-            R.left = pick(R.lr, sh, lng);
-            R.right = pick(R.lr, lng, sh);
-        }
-        for (int y = ya; y < yb; y++) {
-            if (!lower && y == S.up_y1) {   // into the lower half
-                const Side lng = pick(R.lr, R.right, R.left);
-                lower = true;
-                R.lr = S.lr_lo;
-                R.left = pick(R.lr, S.Sl, lng);
-                R.right = pick(R.lr, lng, S.Sl);
-            }
-            R.lower = lower;
+        walk_rows(S, wy0, wy1, [&](int y, const Row &R) {
             Span sp;
-            if (span_of(R, J.V, sp)) {
-                int xa = max(sp.x1, tr.x0), xb = min(sp.x2, tr.x1);
-                if (xa < xb) {
-                    Interp q = sp.q;
-                    q.steps(xa - sp.x1);
-                    const int rowi = (y - tr.y0) * TW - tr.x0;
-                    unsigned long long *row = keys + rowi;
-                    for (int x = xa; x < xb; x++, q.step()) {
-                        const float u = q.u();
-                        float z = q.value(u);
-                        if (z < SW_NEAR)
-                            continue;
-                        frags++;
-                        if (z != z) {
-                            nans++;
-                            continue;
-                        }
-                        if (__float_as_uint(z) >= SW_MAX_Z_BITS)
-                            continue;
-                        const unsigned long long key = vis_key(z, seq);
-                        if (slots) {
-                            if (key < atomicMin(&row[x], key))
-                                put_slots(slots, nslot, rowi + x, seq, code, lower, R.lr, u, sp.pl, sp.pr);
-                        } else {
-                            atomicMin(&row[x], key);
-                        }
-                    }
+            if (!span_of(R, J.V, sp))
+                return;
+            int xa = max(sp.x1, tr.x0), xb = min(sp.x2, tr.x1);
+            if (xa >= xb)
+                return;
+            Interp q = sp.q;
+            q.steps(xa - sp.x1);
+            const int rowi = (y - tr.y0) * TW - tr.x0;
+            unsigned long long *row = keys + rowi;
+            for (int x = xa; x < xb; x++, q.step()) {
+                const float u = q.u();
+                float z = q.value(u);
+                if (z < SW_NEAR)
+                    continue;
+                frags++;
+                if (z != z) {
+                    nans++;
+                    continue;
+                }
+                if (__float_as_uint(z) >= SW_MAX_Z_BITS)
+                    continue;
+                const unsigned long long key = vis_key(z, seq);
+                if (slots) {
+                    if (key < atomicMin(&row[x], key))
+                        put_slots(slots, nslot, rowi + x, seq, code, R.lower, R.lr, u, sp.pl, sp.pr);
+                } else {
+                    atomicMin(&row[x], key);
                 }
             }
-            R.left.steps(1);
-            R.right.steps(1);
-        }
+        });
     }
 }
 
@@ -985,8 +993,12 @@ __device__ uint32_t shade_pixel(const JobDev &J, uint32_t seq, int x, int y, uin
     Span sp;
     // The winner was produced by this very replay, so every step below
     // succeeds; the guards only keep a corrupted key from looping.
-    if ((int)s >= tri_subs<true>(J, t, st) || !setup_subtri(s ? st[1] : st[0], J.V, S, y) || !row_state(S, y, R) ||
-        !span_of(R, J.V, sp) || x < sp.x1 || x >= sp.x2)
+    if ((int)s >= tri_subs<true>(J, t, st))
+        return 0u;
+    SubTri one = st[0];   // a copy, not a runtime index: st stays in registers
+    if (s)
+        one = st[1];
+    if (!setup_subtri(one, J.V, S, y) || !row_state(S, y, R) || !span_of(R, J.V, sp) || x < sp.x1 || x >= sp.x2)
         return 0u;
     Interp q = sp.q;
     q.steps(x - sp.x1);
@@ -1018,7 +1030,10 @@ __device__ uint32_t shade_recorded(const JobDev &J, uint32_t seq, uint32_t tag, 
         SubTri st[2];
         if ((int)s >= tri_subs<true>(J, t, st))
             return 0u;   // not reached: the winner came out of this sub-triangle
-        sort_subtri(s ? st[1] : st[0], S);
+        SubTri one = st[0];
+        if (s)
+            one = st[1];
+        sort_subtri(one, S);
     }
     Row R;
     R.lower = (tag >> 30) & 1u;
@@ -1459,54 +1474,33 @@ __device__ uint64_t layer_walk(const JobDev &J, uint32_t t, const SubTri &st, in
     const bool flat_opaque = (P.color >> 24) == 255;   // tk == 0: the material colour
     const int wy0 = ty0, wy1 = min(ty0 + TH, J.V.vy + J.V.vh), wx1 = min(tx0 + TW, J.V.vx + J.V.vw);
     uint64_t mask = 0ull;
-    int y = wy0;
-    while (y < wy1) {
-        int yend;
-        if (S.has_up && y >= S.up_y0 && y < S.up_y1) {
-            yend = min(S.up_y1, wy1);
-        } else if (S.has_lo && y >= S.lo_y0 && y < S.lo_y1) {
-            yend = min(S.lo_y1, wy1);
-        } else {
-            int nxt = wy1;
-            if (S.has_up && S.up_y0 > y && S.up_y1 > S.up_y0)
-                nxt = min(nxt, S.up_y0);
-            if (S.has_lo && S.lo_y0 > y && S.lo_y1 > S.lo_y0)
-                nxt = min(nxt, S.lo_y0);
-            y = nxt;
-            continue;
-        }
-        Row R;
-        row_state(S, y, R);
-        for (; y < yend; y++) {
-            Span sp;
-            if (span_of(R, J.V, sp)) {
-                const int xa = max(sp.x1, tx0), xb = min(sp.x2, wx1);
-                if (xa < xb) {
-                    Interp q = sp.q;
-                    q.steps(xa - sp.x1);
-                    ScanShade SS;
-                    if (textured)
-                        SS = prepare_scanline(A, R.lower, R.lr, sp.pl, sp.pr);
-                    for (int x = xa; x < xb; x++, q.step()) {
-                        const int p = (y - ty0) * TW + (x - tx0);
-                        const float u = q.u();
-                        z[p] = q.value(u);
-                        mask |= 1ull << p;
-                        bool op = flat_opaque;
-                        if (textured) {
-                            const f2 tc = SS.t_left + SS.t_dir * u;
-                            uint32_t no_count = 0;   // texel reads are counted where the survivors are shaded
-                            op = tk == 1 ? (nearest(X, tc) >> 24) == 255
-                                         : bilinear_channel(bilinear_taps(X, tc, no_count), 3) == 255;
-                        }
-                        opq |= (uint64_t)op << p;
-                    }
-                }
+    walk_rows(S, wy0, wy1, [&](int y, const Row &R) {
+        Span sp;
+        if (!span_of(R, J.V, sp))
+            return;
+        const int xa = max(sp.x1, tx0), xb = min(sp.x2, wx1);
+        if (xa >= xb)
+            return;
+        Interp q = sp.q;
+        q.steps(xa - sp.x1);
+        ScanShade SS;
+        if (textured)
+            SS = prepare_scanline(A, R.lower, R.lr, sp.pl, sp.pr);
+        for (int x = xa; x < xb; x++, q.step()) {
+            const int p = (y - ty0) * TW + (x - tx0);
+            const float u = q.u();
+            z[p] = q.value(u);
+            mask |= 1ull << p;
+            bool op = flat_opaque;
+            if (textured) {
+                const f2 tc = SS.t_left + SS.t_dir * u;
+                uint32_t no_count = 0;   // texel reads are counted where the survivors are shaded
+                op = tk == 1 ? (nearest(X, tc) >> 24) == 255
+                             : bilinear_channel(bilinear_taps(X, tc, no_count), 3) == 255;
             }
-            R.left.steps(1);
-            R.right.steps(1);
+            opq |= (uint64_t)op << p;
         }
-    }
+    });
     if (opaque)
         *opaque = opq;
     return mask;
@@ -1584,7 +1578,10 @@ __device__ __forceinline__ void layer_chunk(const JobDev &J, const uint32_t *lis
         } else if (L.kind == 4u) {
             SubTri st[2];
             const int ns = tri_subs<true>(J, tt, st);
-            for (int sub = 0; sub < ns; sub++) {
+#pragma unroll
+            for (int sub = 0; sub < 2; sub++) {   // unrolled: st is indexed by constants
+                if (sub >= ns)
+                    break;
                 Setup S;
                 Row R;
                 Span sp;
