@@ -121,7 +121,7 @@ struct BatchDev {
     uint32_t n_tiles;         // total tiles over all jobs
     uint32_t *tile_count;     // [n_tiles * SW_TILE_PAD], see SW_TILE_PAD
     uint32_t *tile_offset;    // [n_tiles + 1]
-    uint32_t *tile_order;     // [n_tiles]: tiles by descending bin count (k_scan), the raster's claim order
+    uint32_t *tile_order;     // [n_tiles]: tiles by descending bin count (k_bin_fill's last block), the raster's claim order
     uint32_t *tile_cursor;    // tile_count + 1: the fill ranks
     uint32_t *bins;           // [capacity]
     uint32_t bin_capacity;

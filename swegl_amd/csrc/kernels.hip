@@ -428,20 +428,18 @@ __device__ __forceinline__ void load_counts(const BatchDev &B, uint32_t i0, uint
                           : 0u;
 }
 
-// The scan and the claim order, by one block of NT threads.
+// The exclusive scan of the tile counts into the bin offsets, by one block
+// of NT threads.
 template <int NT>
-__device__ void scan_tiles(const BatchDev &B)
+__device__ void scan_offsets(const BatchDev &B)
 {
     constexpr int NW = NT / 64;
     __shared__ uint32_t wsum[NW];
     __shared__ uint32_t s_carry;
-    __shared__ uint32_t s_hist[SW_ORDER_BUCKETS];
     const uint32_t n = B.n_tiles;
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
     if (threadIdx.x == 0)
         s_carry = 0;
-    for (int b = threadIdx.x; b < SW_ORDER_BUCKETS; b += NT)
-        s_hist[b] = 0;
     __syncthreads();
     uint32_t v[SCAN_PER];
     for (uint32_t base = 0; base < n; base += (uint32_t)NT * SCAN_PER) {
@@ -449,11 +447,8 @@ __device__ void scan_tiles(const BatchDev &B)
         load_counts(B, i0, n, v);
         uint32_t sum = 0;
 #pragma unroll
-        for (int k = 0; k < SCAN_PER; k++) {
+        for (int k = 0; k < SCAN_PER; k++)
             sum += v[k];
-            if (i0 + k < n)
-                atomicAdd(&s_hist[count_bucket(v[k])], 1u);
-        }
         const uint32_t incl = wave_incl_scan(sum);
         if (lane == 63)
             wsum[wave] = incl;
@@ -483,6 +478,31 @@ __device__ void scan_tiles(const BatchDev &B)
         B.tile_offset[n] = s_carry;
         B.ctr[SW_CTR_TOTAL] = s_carry;   // total bin entries
     }
+}
+
+// The raster's claim order, by one block of NT threads: the tiles sorted by
+// a log bucket of their bin counts, the largest bucket first (order within
+// a bucket is arbitrary: the raster's result does not depend on the claim
+// order).
+template <int NT>
+__device__ void order_tiles(const BatchDev &B)
+{
+    __shared__ uint32_t s_hist[SW_ORDER_BUCKETS];
+    const uint32_t n = B.n_tiles;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (int b = threadIdx.x; b < SW_ORDER_BUCKETS; b += NT)
+        s_hist[b] = 0;
+    __syncthreads();
+    uint32_t v[SCAN_PER];
+    for (uint32_t base = 0; base < n; base += (uint32_t)NT * SCAN_PER) {
+        const uint32_t i0 = base + threadIdx.x * SCAN_PER;
+        load_counts(B, i0, n, v);
+#pragma unroll
+        for (int k = 0; k < SCAN_PER; k++)
+            if (i0 + k < n)
+                atomicAdd(&s_hist[count_bucket(v[k])], 1u);
+    }
+    __syncthreads();
     // first place of each bucket, the largest bucket first: wave 0 scans the
     // buckets in descending order, two per lane
     if (wave == 0) {
@@ -493,9 +513,8 @@ __device__ void scan_tiles(const BatchDev &B)
         s_hist[b1] = incl - h1;
     }
     __syncthreads();
-    // every tile takes the next place of its bucket (order within a bucket
-    // is arbitrary: the raster's result does not depend on the claim order);
-    // with one pass the counts are still in registers
+    // every tile takes the next place of its bucket; with one pass the
+    // counts are still in registers
     for (uint32_t base = 0; base < n; base += (uint32_t)NT * SCAN_PER) {
         const uint32_t i0 = base + threadIdx.x * SCAN_PER;
         if (n > (uint32_t)NT * SCAN_PER)
@@ -519,7 +538,8 @@ __device__ void scan_tiles(const BatchDev &B)
 // rectangle is counted here; k_bin_fill, which runs after all marks are
 // set, places only triangles whose three vertices are marked, and the
 // raster checks the marks of big-list triangles.  The last block to finish
-// scans the tile counts and writes the raster's claim order.
+// scans the tile counts into the bin offsets (the claim order is written by
+// k_bin_fill's last block, off this kernel's critical path).
 __global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restrict__ jobs)
 {
     const int jidx = find_job<JT>(B, jobs, blockIdx.x);
@@ -596,14 +616,22 @@ __global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restr
     __syncthreads();
     if (!s_last)
         return;
-    scan_tiles<256>(B);
+    scan_offsets<256>(B);
 }
 
 // ----------------------------------------------------------- k_bin_fill
+// Block 0 writes the raster's claim order instead (the counts are final
+// once k_front has ended): dispatched first, it runs beside the fill, not
+// after it.  Block b > 0 fills for triangle block b - 1.
 __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__restrict__ jobs)
 {
-    const JobDev &J = jobs[find_job<JT>(B, jobs, blockIdx.x)];
-    const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
+    if (blockIdx.x == 0) {
+        order_tiles<256>(B);
+        return;
+    }
+    const uint32_t blk = blockIdx.x - 1;
+    const JobDev &J = jobs[find_job<JT>(B, jobs, blk)];
+    const uint32_t t = (blk - J.tblock0) * 256u + threadIdx.x;
     int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
     uint32_t drawn = 0, big = 0;
     if (t < J.n_tris) {
@@ -1984,13 +2012,15 @@ hipError_t launch_vertex(const BatchDev &B, uint32_t blocks, uint32_t w0, uint32
 hipError_t launch_front(const BatchDev &B, uint32_t blocks, hipStream_t s)
 {
     // at least one block: the last block to finish scans the tile counts
+    // into the bin offsets
     LAUNCH_CHECK(k_front, dim3(blocks ? blocks : 1), dim3(256), 0, s, B, B.jobs);
     return hipSuccess;
 }
 hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s)
 {
-    if (blocks)
-        LAUNCH_CHECK(k_bin_fill, dim3(blocks), dim3(256), 0, s, B, B.jobs);
+    // one more block than the fill needs: the first one writes the claim order
+    if (B.n_tiles)
+        LAUNCH_CHECK(k_bin_fill, dim3(blocks + 1), dim3(256), 0, s, B, B.jobs);
     return hipSuccess;
 }
 template <bool G>
