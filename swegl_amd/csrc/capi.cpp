@@ -86,11 +86,16 @@ struct swegl_target {
 };
 
 // Per-viewport scratch of the context lane.  The front end's buffers come in
-// two sets (the eager path's parity, see swegl_ctx), so that one render's
-// front end can run while the previous render's raster still reads its own.
+// kFrontSets sets (the eager path's pipeline, see swegl_ctx), so that the
+// front ends of the next renders can run while earlier renders' rasters
+// still read their own.
+static const int kFrontSets = 3;
+struct FrontScratch {
+    DevBuf ndc, yes, rect, big;
+};
 struct ViewScratch {
-    DevBuf ndc, yes, rect, big, dof, tflat;
-    DevBuf ndc1, yes1, rect1, big1;
+    FrontScratch front[kFrontSets];
+    DevBuf dof, tflat;
 };
 
 struct swegl_scene {
@@ -198,7 +203,7 @@ struct GraphEntry {
 // one region whatever scenes pass through it.
 struct Lane {
     hipStream_t stream = nullptr;
-    Resources res, res1;   // res1: the context lane's second set (parity 1)
+    Resources res[kFrontSets];   // the context lane uses one set per pipeline slot; others res[0]
     DevBuf region;
     StageSlot stage[4];
     int stage_next = 0;
@@ -232,17 +237,19 @@ struct swegl_ctx {
     // The eager path's two-stream pipeline.  A render with one launch wave
     // runs its front end (job table upload, k_vertex, k_front, k_bin_fill)
     // on fstream and its raster and depth of field on the context stream,
-    // which waits for that front end.  Renders alternate between two sets
-    // of front-end buffers (parity), so the next render's front end only
-    // waits for the raster that last used its set, two renders back, and
-    // runs while the previous render's raster and depth of field finish.
-    // Any other work on the context stream (uploads, pose updates, target
-    // writes, other render paths) bumps op_seq; the next pipelined render
-    // then starts its front end after everything on the context stream.
+    // which waits for that front end.  Renders take the kFrontSets sets of
+    // front-end buffers in turn (slot), so a render's front end only waits
+    // for the render that last used its set, kFrontSets renders back: the
+    // front ends run ahead, in the gaps the rasters and depth-of-field
+    // passes of the renders before leave, and are done before their own
+    // raster is due.  Any other work on the context stream (uploads, pose
+    // updates, target writes, other render paths) bumps op_seq; the next
+    // pipelined render then starts its front end after everything on the
+    // context stream.
     hipStream_t fstream = nullptr;
-    hipEvent_t f_done[2] = {nullptr, nullptr}, r_done[2] = {nullptr, nullptr}, tail = nullptr;
-    bool r_recorded[2] = {false, false};
-    int parity = 0;
+    hipEvent_t f_done[kFrontSets] = {}, r_done[kFrontSets] = {}, tail = nullptr;
+    bool r_recorded[kFrontSets] = {};
+    int front_slot = 0;   // the next render's set of front-end buffers
     uint64_t op_seq = 1, pipe_seq = 0;   // equal: only pipelined renders since the last one
     std::vector<char> noalloc_tables;      // host job tables of the allocation-free path
     hipEvent_t noalloc_done = nullptr;     // their last upload
@@ -349,7 +356,8 @@ int Lane::init(bool own)
     if (own)
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (StageSlot &st : stage)
-        HIPCHK(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+        // only tells the host that the copy out of the slot has ended: no fence
+        HIPCHK(hipEventCreateWithFlags(&st.done, hipEventDisableTiming | hipEventDisableSystemFence));
     return SWEGL_OK;
 }
 
@@ -430,11 +438,16 @@ extern "C" int swegl_ctx_create(int device, swegl_ctx **out)
         return src;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&c->fstream, hipStreamNonBlocking));
-    for (int k = 0; k < 2; k++) {
-        HIPCHK(hipEventCreateWithFlags(&c->f_done[k], hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&c->r_done[k], hipEventDisableTiming));
+    // These events order work between the device's own streams: each kernel
+    // ends with its own device-scope release, so they skip the system-scope
+    // fence (an L2 write-back and invalidate on every XCD, ~10 us) that the
+    // default event record adds for the host's and other devices' sake.
+    const unsigned dev_ev = hipEventDisableTiming | hipEventDisableSystemFence;
+    for (int k = 0; k < kFrontSets; k++) {
+        HIPCHK(hipEventCreateWithFlags(&c->f_done[k], dev_ev));
+        HIPCHK(hipEventCreateWithFlags(&c->r_done[k], dev_ev));
     }
-    HIPCHK(hipEventCreateWithFlags(&c->tail, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->tail, dev_ev));
     HIPCHK(hipHostMalloc((void **)&c->h_stats, SW_CTR_CONTROL * sizeof(uint32_t), hipHostMallocDefault));
     HIPCHK(hipEventCreateWithFlags(&c->noalloc_done, hipEventDisableTiming));
     c->lane0.stream = c->stream;
@@ -463,9 +476,12 @@ extern "C" int swegl_ctx_destroy(swegl_ctx *c)
     if (c->h_stage)
         (void)hipHostFree(c->h_stage);
     c->graphs.clear();
-    for (hipEvent_t e : {c->f_done[0], c->f_done[1], c->r_done[0], c->r_done[1], c->tail})
-        if (e)
-            (void)hipEventDestroy(e);
+    for (int k = 0; k < kFrontSets; k++)
+        for (hipEvent_t e : {c->f_done[k], c->r_done[k]})
+            if (e)
+                (void)hipEventDestroy(e);
+    if (c->tail)
+        (void)hipEventDestroy(c->tail);
     if (c->fstream)
         (void)hipStreamDestroy(c->fstream);
     (void)hipStreamDestroy(c->stream);
@@ -955,8 +971,11 @@ extern "C" int swegl_scene_bytes(swegl_ctx *c, swegl_scene *s, uint64_t *bytes)
     if (!c || !s || !bytes)
         return fail(SWEGL_EINVAL, "null argument");
     uint64_t b = s->resident;
-    for (auto &v : s->views)
-        b += v->ndc.n + v->yes.n + v->rect.n + v->big.n + v->dof.n + v->tflat.n;
+    for (auto &v : s->views) {
+        b += v->dof.n + v->tflat.n;
+        for (const FrontScratch &f : v->front)
+            b += f.ndc.n + f.yes.n + f.rect.n + f.big.n;
+    }
     *bytes = b;
     return SWEGL_OK;
 }
@@ -1085,7 +1104,7 @@ static size_t job_scratch_sizes(const swegl_scene *s, const swegl_viewport_desc 
     return sz[0] + sz[1] + sz[2] + sz[3] + sz[4] + sz[5];
 }
 static int job_scratch(Lane &L, Arena *arena, swegl_scene *s, int use, const swegl_viewport_desc &v,
-                       JobScratch &out, int parity)
+                       JobScratch &out, int slot)
 {
     size_t sz[6];
     job_scratch_sizes(s, v, sz);
@@ -1097,8 +1116,8 @@ static int job_scratch(Lane &L, Arena *arena, swegl_scene *s, int use, const swe
                                           "the %zu bytes of the fixed scratch array", arena->cap);
     } else {
         ViewScratch &vs = *L.scratch(s, use);
-        DevBuf *b[6] = {parity ? &vs.ndc1 : &vs.ndc, parity ? &vs.yes1 : &vs.yes, parity ? &vs.rect1 : &vs.rect,
-                        parity ? &vs.big1 : &vs.big, &vs.dof, &vs.tflat};
+        FrontScratch &f = vs.front[slot];
+        DevBuf *b[6] = {&f.ndc, &f.yes, &f.rect, &f.big, &vs.dof, &vs.tflat};
         for (int k = 0; k < 6; k++)
             if (sz[k]) {
                 int rc = b[k]->ensure(sz[k]);
@@ -1116,7 +1135,7 @@ static int job_scratch(Lane &L, Arena *arena, swegl_scene *s, int use, const swe
     return SWEGL_OK;
 }
 
-static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wave &W, Arena *arena, int parity)
+static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wave &W, Arena *arena, int slot)
 {
     const uint32_t nj = (uint32_t)jobs.size();
     W.jd.assign(nj, JobDev{});
@@ -1127,7 +1146,7 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         swegl_scene *s = jobs[j].scene;
         const swegl_viewport_desc &v = *jobs[j].vp;
         JobScratch vs;
-        int rc = job_scratch(L, arena, s, scene_uses[s]++, v, vs, parity);
+        int rc = job_scratch(L, arena, s, scene_uses[s]++, v, vs, slot);
         if (rc)
             return rc;
         const bool dof = v.post_shader != SWEGL_POST_NULL;
@@ -1339,7 +1358,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bo
 }
 
 static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::vector<Wave> &waves,
-                   Arena *arena = nullptr, int parity = 0)
+                   Arena *arena = nullptr, int slot = 0)
 {
     HIPCHK(hipSetDevice(c->device));
     for (const JobIn &j : all) {
@@ -1374,7 +1393,7 @@ static int prepare(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, std::ve
         if (groups[w].empty())
             continue;
         waves.emplace_back();
-        int rc = build_wave(c, L, groups[w], waves.back(), arena, parity);
+        int rc = build_wave(c, L, groups[w], waves.back(), arena, slot);
         if (rc)
             return rc;
     }
@@ -1446,21 +1465,21 @@ static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swe
         if (waves.empty())
             return no_waves(stats);
     } else {
-        // the context lane: the front-end buffers of this render's parity
-        const int p = c->parity;
-        c->parity ^= 1;
+        // the context lane: the front-end buffers of this render's slot
+        const int p = c->front_slot;
+        c->front_slot = (p + 1) % kFrontSets;
         if ((rc = prepare(c, L, all, waves, nullptr, p)))
             return rc;
         if (waves.empty())
             return no_waves(stats);
-        Resources &RS = p ? L.res1 : L.res;
+        Resources &RS = L.res[p];
         if ((rc = size_resources(RS, waves)))
             return rc;
         R = res_ptr(RS);
         if (waves.size() == 1 && !waves[0].layers && !c->prof) {
             // Pipelined (not while kernels are timed, so that each stage's
             // events time that stage alone): the front end on fstream, after
-            // the raster that last used this parity's buffers, or after
+            // the raster that last used this slot's buffers, or after
             // everything on the context stream when other work was queued
             // there since the last pipelined render
             hipStream_t fs = c->fstream;
@@ -1582,7 +1601,7 @@ static int evict_graphs(swegl_ctx *c)
 
 static int render_jobs_graph(swegl_ctx *c, const std::vector<JobIn> &all, swegl_frame_stats *stats)
 {
-    c->op_seq++;   // the per-viewport scratch of parity 0, on the context stream
+    c->op_seq++;   // the per-viewport scratch of slot 0, on the context stream
     std::vector<Wave> waves;
     int rc = prepare(c, c->lane0, all, waves);
     if (rc)
@@ -1687,7 +1706,7 @@ extern "C" int swegl_debug_vertex_state(swegl_ctx *c, swegl_scene *s, const sweg
 {
     if (!c || !s || !vp || (s->n_vertices && (!v_world || !normal_world || !v_viewport || !yes)))
         return fail(SWEGL_EINVAL, "null argument");
-    const int parity = c->parity;   // the set of front-end buffers the render below uses
+    const int slot = c->front_slot;   // the set of front-end buffers the render below uses
     int rc = swegl_render(c, s, vp, 1, nullptr);
     if (rc)
         return rc;
@@ -1702,8 +1721,8 @@ extern "C" int swegl_debug_vertex_state(swegl_ctx *c, swegl_scene *s, const sweg
     std::vector<uint8_t> y(V);
     HIPCHK(hipMemcpy(w.data(), s->world.p, V * sizeof(float4), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(n.data(), s->nworld.p, V * sizeof(float4), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(f.data(), (parity ? vs.ndc1 : vs.ndc).p, V * sizeof(float4), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(y.data(), (parity ? vs.yes1 : vs.yes).p, V, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(f.data(), vs.front[slot].ndc.p, V * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(y.data(), vs.front[slot].yes.p, V, hipMemcpyDeviceToHost));
     float m00, m03, m11, m13;
     hm::viewport_matrix(vp->x, vp->y, vp->w, vp->h, &m00, &m03, &m11, &m13);
     for (size_t i = 0; i < V; i++) {

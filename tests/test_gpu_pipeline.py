@@ -1,7 +1,7 @@
 """Renders queued back to back without synchronisation.  swegl_render runs a
-render's front end (k_vertex, k_front, k_bin_fill) on a second stream with
-the second of two sets of front-end buffers, so that it overlaps the
-previous render's raster and depth of field.  Every frame of such a queue,
+render's front end (k_vertex, k_front, k_bin_fill) on a second stream, with
+one of three sets of front-end buffers taken in turn, so that it overlaps
+the previous renders' rasters and depth of field.  Every frame of such a queue,
 interleaved with pose updates, graph replays and renders with statistics,
 must equal the same render done alone, bit for bit."""
 import numpy as np
