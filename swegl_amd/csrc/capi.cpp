@@ -1046,6 +1046,7 @@ struct Wave {
     bool generic = false;     // some job uses a shader other than Phong + bilinear
     bool layers = false;      // the wave's (single) job has transparency layers
     uint64_t cap = 0;
+    uint32_t tile_cap = 0;    // entries per tile of k_front's fixed layout (BatchDev::tile_cap), 0: none
     uint32_t big_tiles = 0;   // bin limit of this wave (BatchDev::big_tiles)
     size_t job_offset = 0;   // byte offset of this wave's table in Resources::jobs
 };
@@ -1237,6 +1238,16 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
     W.cap = std::max<uint64_t>(4096, tris_total * W.big_tiles);
     if (W.cap > 0xFFFFFFF0ull)
         return fail(SWEGL_ERANGE, "too many triangles in one batch");
+    // k_front's fixed layout (the context lane's buffers, no layers): room
+    // for eight times a tile's share of 1.25 entries per triangle, at least
+    // 512; a frame whose busiest tile needs more takes the filled layout
+    W.tile_cap = 0;
+    if (!arena && !W.layers && W.tiles && tris_total < 0x80000000ull) {   // (SW_BIN_CHECK is the top bit)
+        const uint64_t per = (tris_total * 10 + W.tiles - 1) / W.tiles;   // 8 x 1.25 entries per triangle
+        const uint64_t tc = (std::max<uint64_t>(512, per) + 63) & ~(uint64_t)63;
+        if ((uint64_t)W.tiles * tc <= 0xFFFFFFF0ull)
+            W.tile_cap = (uint32_t)tc;
+    }
     return SWEGL_OK;
 }
 
@@ -1265,7 +1276,7 @@ static void resource_sizes(std::vector<Wave> &waves, size_t sz[5])
     for (Wave &W : waves) {
         tiles = std::max(tiles, W.tiles);
         n_ctr = std::max(n_ctr, W.n_ctr);
-        cap = std::max(cap, W.cap);
+        cap = std::max(cap, std::max<uint64_t>(W.cap, (uint64_t)W.tiles * W.tile_cap));
     }
     sz[0] = job_bytes;
     sz[1] = (size_t)tiles * SW_TILE_PAD * 4 + 8;
@@ -1323,6 +1334,7 @@ static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bo
     B.tile_cursor = R.tile_count + 1;
     B.bins = R.bins;
     B.bin_capacity = (uint32_t)W.cap;
+    B.tile_cap = W.tile_cap;
     B.ctr = R.ctr;
     B.big_tiles = W.big_tiles;
     B.stats = want_stats ? 1u : 0u;
@@ -1570,7 +1582,8 @@ static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
         k += "|" + std::to_string(W.tw) + "x" + std::to_string(W.th) + ":" + std::to_string(W.jd.size()) + ":" +
              std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
              "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap) + "," +
-             std::to_string(W.rmax) + "," + std::to_string(W.big_tiles) + (W.generic ? "g" : "") +
+             std::to_string(W.rmax) + "," + std::to_string(W.big_tiles) + "," + std::to_string(W.tile_cap) +
+             (W.generic ? "g" : "") +
              (W.layers ? "L" : "");
         // the layer waves' flatten and copy grids are sized by the viewport's pixels
         if (W.layers)

@@ -128,6 +128,11 @@ struct BatchDev {
     uint32_t big_tiles;       // per-triangle bin limit; larger ones go to the big list
     uint32_t *ctr;            // counter words, layout below
     uint32_t stats;           // 1: accumulate the statistics counters
+    // > 0: k_front places every entry at bins[g * tile_cap + rank] as it
+    // counts it (marks unchecked), and k_bin_fill and the offsets scan run
+    // only when a tile holds more entries than that (SW_CTR_OVERFLOW set);
+    // 0: the counted, scanned and filled layout alone
+    uint32_t tile_cap;
 };
 
 // Counter block: [0, 1024) statistics as 64 slots x 8 u64 (slot = block
@@ -144,6 +149,11 @@ struct BatchDev {
 // index k mod SW_TICKET_GROUPS, then one of the finished groups, each on
 // its own 128-byte line (same-address atomics serialise where they execute)
 #define SW_TICKET_GROUPS 32
+#define SW_CTR_OVERFLOW (SW_CTR_TOTAL + 32)   // nonzero: a tile passed tile_cap
+// A fixed-layout bin entry with this bit set names a triangle that failed its
+// own visibility test: it is drawn only if other triangles marked all three
+// of its vertices, which the raster looks up (a job holds < 2^31 triangles)
+#define SW_BIN_CHECK 0x80000000u
 #define SW_CTR_TICKETS (SW_CTR_TOTAL + 64)
 #define SW_CTR_BIG0 (SW_CTR_TICKETS + 32 * (SW_TICKET_GROUPS + 1))
 // statistic indices within a slot

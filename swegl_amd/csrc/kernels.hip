@@ -547,6 +547,7 @@ __global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restr
     const uint32_t t = (blockIdx.x - J.tblock0) * 256u + threadIdx.x;
     uint32_t big = 0;
     int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
+    bool self_marked = false;   // marked its own vertices: drawn whatever other triangles do
     if (t < J.n_tris) {
         const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
         const f3 a = ld3(J.ndc, i0), b = ld3(J.ndc, i1), c = ld3(J.ndc, i2);
@@ -555,6 +556,7 @@ __global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restr
             J.yes[i0] = 1;
             J.yes[i1] = 1;
             J.yes[i2] = 1;
+            self_marked = true;
         }
         Extent E;
         bool exact;
@@ -581,7 +583,9 @@ __global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restr
         if (big)
             J.big[pos] = t;
     }
-    // one aggregated atomic per distinct tile per wave and round
+    // one aggregated atomic per distinct tile per wave and round; with a
+    // tile capacity the count's old value places the entry
+    bool overflow = false;
     for (int k = 0; __any(k < ntiles); k++) {
         const bool act = k < ntiles;
         uint32_t key = 0;
@@ -590,8 +594,17 @@ __global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restr
             const int ky = k == 0 ? 0 : k / w, kx = k == 0 ? 0 : k % w;   // k is the same in every lane
             key = J.tile_base + (ty0 + ky) * J.tiles_x + (tx0 + kx);
         }
-        wave_add<false>(B.tile_count, key * SW_TILE_PAD, act);
+        if (B.tile_cap) {
+            const uint32_t rank = wave_add<true>(B.tile_count, key * SW_TILE_PAD, act);
+            if (act && rank < B.tile_cap)   // top bit: the raster must look at the marks
+                B.bins[(size_t)key * B.tile_cap + rank] = self_marked ? t : t | SW_BIN_CHECK;
+            overflow |= act && rank >= B.tile_cap;
+        } else {
+            wave_add<false>(B.tile_count, key * SW_TILE_PAD, act);
+        }
     }
+    if (__any(overflow) && __lane_id() == 0)
+        atomicOr(B.ctr + SW_CTR_OVERFLOW, 1u);
     // The last block to finish scans.  The counts are device-scope atomics,
     // performed at the memory side, not in an XCD's L2: each wave waits
     // for its own to complete before the block takes its ticket, and the
@@ -616,7 +629,9 @@ __global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restr
     __syncthreads();
     if (!s_last)
         return;
-    scan_offsets<256>(B);
+    // the offsets serve the filled layout only
+    if (!B.tile_cap || __hip_atomic_load(B.ctr + SW_CTR_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        scan_offsets<256>(B);
 }
 
 // ----------------------------------------------------------- k_bin_fill
@@ -629,6 +644,11 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__re
         order_tiles<256>(B);
         return;
     }
+    // k_front placed every entry unless a tile overflowed: then only the
+    // statistics, if asked for, are counted here
+    const bool place = !B.tile_cap || B.ctr[SW_CTR_OVERFLOW] != 0;
+    if (!place && !B.stats)
+        return;
     const uint32_t blk = blockIdx.x - 1;
     const JobDev &J = jobs[find_job<JT>(B, jobs, blk)];
     const uint32_t t = (blk - J.tblock0) * 256u + threadIdx.x;
@@ -662,6 +682,10 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__re
     uint32_t entries = 0;
     for (int k = 0; __any(k < ntiles); k++) {
         const bool act = k < ntiles;
+        if (!place) {   // (wave-uniform)
+            entries += act ? 1u : 0u;
+            continue;
+        }
         uint32_t key = 0;
         if (act) {
             const int w = tx1 - tx0 + 1;
@@ -831,12 +855,16 @@ __device__ __forceinline__ void walk_rows(const Setup &S, int wy0, int wy1, RowF
     }
 }
 
+// check_mark: t came from a list that holds unmarked triangles too (the
+// marks are read beside the vertices, the triangle skipped when unmarked).
 __device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, int wy1,
                              unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans,
-                             unsigned long long *slots = nullptr, int nslot = 0)
+                             unsigned long long *slots = nullptr, int nslot = 0, bool check_mark = false)
 {
     SubTri st[2];
     const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
+    if (check_mark && !((ro(J.yes)[i0] != 0) & (ro(J.yes)[i1] != 0) & (ro(J.yes)[i2] != 0)))
+        return;
     int ns = tri_subs_v<true>(J, i0, i1, i2, st);
 #pragma unroll
     for (int s = 0; s < 2; s++) {
@@ -1074,7 +1102,8 @@ __device__ uint32_t shade_recorded(const JobDev &J, uint32_t seq, uint32_t tag, 
 
 template <int TW, int TH, bool GENERIC>
 __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__restrict__ jobs, uint32_t g,
-                                            unsigned long long *keys, uint32_t *s_cnt, unsigned long long *slots)
+                                            unsigned long long *keys, uint32_t *s_cnt, unsigned long long *slots,
+                                            bool fixed)
 {
     constexpr int NSLOT = TW * TH;
     const int jidx = find_job<JTILE>(B, jobs, g);
@@ -1099,17 +1128,30 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
     // are dealt to the lanes, so a large triangle does not leave one lane
     // looping over its pixels while the others idle.
     __shared__ uint32_t s_tri[256], s_rows[256], s_first[256], s_wsum[4];
-    // entries placed by k_bin_fill (marked triangles only; the count also
-    // held unmarked ones), then the job's big list
-    const uint32_t nb = B.tile_cursor[g * SW_TILE_PAD], off = B.tile_offset[g];
+    // The tile's entries: placed by k_front at g * tile_cap (every triangle
+    // whose rectangle holds the tile; marks are checked here), or, when a
+    // tile passed that capacity, by k_bin_fill at the scanned offset (marked
+    // triangles only; the count also held unmarked ones); then the job's big
+    // list.
+    uint32_t nb, off;
+    if (fixed) {
+        nb = min(B.tile_count[g * SW_TILE_PAD], B.tile_cap);
+        off = g * B.tile_cap;
+    } else {
+        nb = B.tile_cursor[g * SW_TILE_PAD];
+        off = B.tile_offset[g];
+    }
     const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
     if (nb + nbig > 256) {
         // enough triangles to fill the lanes: one lane per triangle, all rows
         for (uint32_t e = threadIdx.x; e < nb + nbig; e += 256) {
             uint32_t t;
+            bool check = false;
             if (e < nb) {
                 t = ro(B.bins)[off + e];
+                check = (t & SW_BIN_CHECK) != 0;   // never set in the filled layout
+                t &= ~SW_BIN_CHECK;
             } else {
                 t = ro(J.big)[e - nb];
                 const uint2 r = ro2(J.rect, t);
@@ -1117,16 +1159,19 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
                     ty > (int)(r.y >> 16) || !marked(J, t))
                     continue;
             }
-            raster_depth(J, t, tr, tr.y0, tr.y1, keys, TW, frags, nans, slots, NSLOT);
+            raster_depth(J, t, tr, tr.y0, tr.y1, keys, TW, frags, nans, slots, NSLOT, check);
         }
     } else {
     for (uint32_t base = 0; base < nb + nbig; base += 256) {
         const uint32_t e = base + threadIdx.x;
         uint32_t t = 0xFFFFFFFFu, nch = 0;
         int ry0 = tr.y0, ry1 = tr.y1;
+        bool check = false;
         if (e < nb + nbig) {
             if (e < nb) {
                 t = ro(B.bins)[off + e];
+                check = (t & SW_BIN_CHECK) != 0;
+                t &= ~SW_BIN_CHECK;
             } else {
                 t = ro(J.big)[e - nb];
                 const uint2 r = ro2(J.rect, t);
@@ -1138,6 +1183,11 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
         if (t != 0xFFFFFFFFu) {
             // rows [ceil(ymin), ceil(ymax)) are exactly the rows fill_triangle_2 visits for an
             // unclipped triangle; clipped or non-finite ones keep all rows of the tile
+            const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
+            if (check && !((ro(J.yes)[i0] != 0) & (ro(J.yes)[i1] != 0) & (ro(J.yes)[i2] != 0)))
+                t = 0xFFFFFFFFu;   // unmarked: no chunks
+        }
+        if (t != 0xFFFFFFFFu) {
             const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
             const f3 a = frustum_to_screen(J.V, ld3(J.ndc, i0)), b = frustum_to_screen(J.V, ld3(J.ndc, i1)),
                      c = frustum_to_screen(J.V, ld3(J.ndc, i2));
@@ -1273,6 +1323,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // index mod 8), then the others in turn; a queue seen exhausted is never
     // asked again, so every workgroup stops after at most n_tiles + 8 claims
     // in total.
+    // which layout the bins have this frame (the same for every tile)
+    const bool fixed = B.tile_cap != 0 && B.ctr[SW_CTR_OVERFLOW] == 0;
     const uint32_t T = B.n_tiles, home = blockIdx.x % SW_RASTER_QUEUES;
     uint32_t tried = 0;   // thread 0 only: queues found exhausted
     for (;;) {
@@ -1298,7 +1350,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const uint32_t g = s_tile;
         if (g == 0xFFFFFFFFu)
             break;
-        raster_tile<TW, TH, GENERIC>(B, jobs, g, keys, s_cnt, SLOTS ? slots : nullptr);
+        raster_tile<TW, TH, GENERIC>(B, jobs, g, keys, s_cnt, SLOTS ? slots : nullptr, fixed);
         __syncthreads();   // tile done with keys and s_tile before the next claim
     }
     __syncthreads();
