@@ -267,6 +267,15 @@ struct Extent {
     int y0, y1;   // [y0, y1)
     int x0, x1;   // [x0, x1]
 };
+// Columns of a triangle's tile rectangle: the vertex x range widened by
+// SW_RECT_MARGIN pixels plus a bound of the DDA's accumulated rounding
+// ((rows + 8) add errors of at most 2^-20 |x|max each, 16 times the float
+// half-ulp).  A covered pixel x satisfies ceil(left.x) <= x < ceil(right.x)
+// with both sides within the vertex x range up to that rounding, so the
+// bound holds with no margin at all; the margin is spare.  A brute-force
+// check against the rasteriser on adversarial triangles pins it.
+#define SW_RECT_MARGIN 0.5f
+
 __device__ bool sub_extent(const Setup &S, const ViewXf &V, Extent &E)
 {
     int ya = INT32_MAX, yb = INT32_MIN;
@@ -287,7 +296,7 @@ __device__ bool sub_extent(const Setup &S, const ViewXf &V, Extent &E)
     // The DDA reaches row k after k float adds; each add is off by at most
     // half an ulp of the largest |x| on the walk.  Widen by that bound.
     float X = fmaxf(fmaxf(fabsf(xmin), fabsf(xmax)), 1.0f);
-    float slack = 2.0f + (float)(yb - ya + 8) * X * (1.0f / 1048576.0f);
+    float slack = SW_RECT_MARGIN + (float)(yb - ya + 8) * X * (1.0f / 1048576.0f);
     if (!(xmin == xmin) || !(xmax == xmax) || X > 1e30f) {
         E.x0 = V.vx;
         E.x1 = V.vx + V.vw - 1;
@@ -332,7 +341,7 @@ __device__ bool quick_extent(const JobDev &J, f3 a, f3 b, f3 c, Extent &E, uint3
     E.y1 = y1;
     const float xmin = fminf(a.x, fminf(b.x, c.x)), xmax = fmaxf(a.x, fmaxf(b.x, c.x));
     const float X = fmaxf(fmaxf(fabsf(xmin), fabsf(xmax)), 1.0f);
-    const float slack = 2.0f + (float)(iy2 - iy0 + 8) * X * (1.0f / 1048576.0f);
+    const float slack = SW_RECT_MARGIN + (float)(iy2 - iy0 + 8) * X * (1.0f / 1048576.0f);
     const float fx0 = floorf(xmin - slack), fx1 = ceilf(xmax + slack);
     E.x0 = fx0 < (float)J.V.vx ? J.V.vx : (int)fx0;
     E.x1 = fx1 > (float)(J.V.vx + J.V.vw - 1) ? J.V.vx + J.V.vw - 1 : (int)fx1;
