@@ -248,7 +248,11 @@ struct swegl_ctx {
     // context stream.
     hipStream_t fstream = nullptr;
     hipEvent_t f_done[kFrontSets] = {}, r_done[kFrontSets] = {}, tail = nullptr;
-    bool r_recorded[kFrontSets] = {};
+    bool r_recorded[kFrontSets] = {}, f_recorded[kFrontSets] = {};
+    // pinned host copy of each slot's job table, reused once that slot's
+    // last front end (which copied it) is done: f_done, no event of its own
+    void *job_pinned[kFrontSets] = {};
+    size_t job_pinned_n[kFrontSets] = {};
     int front_slot = 0;   // the next render's set of front-end buffers
     uint64_t op_seq = 1, pipe_seq = 0;   // equal: only pipelined renders since the last one
     std::vector<char> noalloc_tables;      // host job tables of the allocation-free path
@@ -482,6 +486,9 @@ extern "C" int swegl_ctx_destroy(swegl_ctx *c)
                 (void)hipEventDestroy(e);
     if (c->tail)
         (void)hipEventDestroy(c->tail);
+    for (void *h : c->job_pinned)
+        if (h)
+            (void)hipHostFree(h);
     if (c->fstream)
         (void)hipStreamDestroy(c->fstream);
     (void)hipStreamDestroy(c->stream);
@@ -1353,11 +1360,11 @@ static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bo
         fs = st;
     STAGE("vertex", fs, launch_vertex(B, W.vblk, zero0, W.n_ctr - zero0, fs));
     STAGE("front", fs, launch_front(B, W.tblk, fs));
-    STAGE("bin_fill", fs, launch_bin_fill(B, W.tblk, fs));
-    if (fs != st) {
-        HIPCHK(hipEventRecord(fdone, fs));
+    // with two streams, the fill's own completion records fdone (no
+    // separate marker on fstream, which would hold up the next front end)
+    STAGE("bin_fill", fs, launch_bin_fill(B, W.tblk, fs, fs != st ? fdone : nullptr));
+    if (fs != st)
         HIPCHK(hipStreamWaitEvent(st, fdone, 0));
-    }
     if (W.layers) {
         const uint32_t pixels = (uint32_t)(W.jd[0].V.vw * W.jd[0].V.vh);
         STAGE("raster", st, launch_layers(B, W.generic, pixels, st));
@@ -1503,8 +1510,22 @@ static int render_jobs(swegl_ctx *c, Lane &L, const std::vector<JobIn> &all, swe
             }
             c->r_recorded[p] = false;
             c->pipe_seq = c->op_seq - 1;   // until this render is fully queued
-            if ((rc = upload(L, R.jobs + waves[0].job_offset, waves[0].jd.data(),
-                             waves[0].jd.size() * sizeof(JobDev), fs)) ||
+            const size_t jb = waves[0].jd.size() * sizeof(JobDev);
+            if (c->f_recorded[p])
+                HIPCHK(hipEventSynchronize(c->f_done[p]));   // the slot's last copy has ended
+            if (c->job_pinned_n[p] < jb) {
+                if (c->job_pinned[p])
+                    (void)hipHostFree(c->job_pinned[p]);
+                c->job_pinned[p] = nullptr;
+                c->job_pinned_n[p] = 0;
+                HIPCHK(hipHostMalloc(&c->job_pinned[p], std::max<size_t>(jb, 4096), hipHostMallocDefault));
+                c->job_pinned_n[p] = std::max<size_t>(jb, 4096);
+            }
+            memcpy(c->job_pinned[p], waves[0].jd.data(), jb);
+            c->f_recorded[p] = true;
+            hipError_t ce = hipMemcpyAsync(R.jobs + waves[0].job_offset, c->job_pinned[p], jb,
+                                           hipMemcpyHostToDevice, fs);
+            if ((rc = ce != hipSuccess ? fail(SWEGL_EIO, "job table copy: %s", hipGetErrorString(ce)) : 0) ||
                 (rc = launch_wave(c, L, R, waves[0], true, false, stats != nullptr, fs, c->f_done[p]))) {
                 // whatever reached fstream is waited for by the context stream
                 (void)hipEventRecord(c->f_done[p], fs);

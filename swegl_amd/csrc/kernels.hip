@@ -9,6 +9,7 @@
 // Every kernel works on a batch of jobs (one job = one viewport of one
 // scene); a block finds its job by binary search over the job table, so the
 // plain path is simply a batch of one.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "jobs.h"
@@ -2077,11 +2078,17 @@ hipError_t launch_front(const BatchDev &B, uint32_t blocks, hipStream_t s)
     LAUNCH_CHECK(k_front, dim3(blocks ? blocks : 1), dim3(256), 0, s, B, B.jobs);
     return hipSuccess;
 }
-hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s)
+hipError_t launch_bin_fill(const BatchDev &B, uint32_t blocks, hipStream_t s, hipEvent_t done)
 {
     // one more block than the fill needs: the first one writes the claim order
-    if (B.n_tiles)
-        LAUNCH_CHECK(k_bin_fill, dim3(blocks + 1), dim3(256), 0, s, B, B.jobs);
+    if (!B.n_tiles)
+        return done ? hipEventRecord(done, s) : hipSuccess;
+    if (done) {
+        // the kernel's own completion signals `done`: no marker after it
+        hipExtLaunchKernelGGL(k_bin_fill, dim3(blocks + 1), dim3(256), 0, s, nullptr, done, 0, B, B.jobs);
+        return hipGetLastError();
+    }
+    LAUNCH_CHECK(k_bin_fill, dim3(blocks + 1), dim3(256), 0, s, B, B.jobs);
     return hipSuccess;
 }
 template <bool G>
