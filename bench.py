@@ -441,8 +441,8 @@ def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fr
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tile", default=None, help="WxH raster tile, e.g. 32x32")
     ap.add_argument("--raster-groups", type=int, default=0, help="raster workgroups per CU (0 = tuned)")
