@@ -134,3 +134,34 @@ def test_crowded_tiles_match_the_oracle(ctx, back):
         assert np.array_equal(z, got.zbuffer.view(np.uint32).reshape(z.shape))
         assert np.array_equal(tid, got.tid.reshape(tid.shape))
     d.close()
+
+
+def test_dof_and_plain_renders_into_one_surface(ctx):
+    """A viewport with depth of field and one without, on one scene, writing
+    the same colour surface and depth plane, queued in turns without a sync:
+    one writes the surface from its depth-of-field pass, the other from its
+    raster, and the targets hold the last render."""
+    s, vd = scenes.heightfield_scene(3, n=300, width=640, height=360, dof=True)
+    _, vp = scenes.heightfield_scene(3, n=300, width=640, height=360, dof=False)
+    d = DeviceScene(ctx, SceneDescHolder(s))
+    a = Job(ctx, d, vd[0], 0.0)
+    b = Job(ctx, d, vp[0], 0.0)
+    b.col, b.z, b.tid = a.col, a.z, a.tid
+    b.desc[0] = vp[0].desc(a.col, a.z, a.tid)
+    want = {}
+    for name, j in (("dof", a), ("plain", b)):
+        a.clear()
+        ctx.sync()
+        j.render(ctx)
+        ctx.sync()
+        want[name] = a.read()
+    assert not np.array_equal(want["dof"][0], want["plain"][0])
+    for order in (("dof", "plain"), ("plain", "dof"), ("dof", "dof", "plain"), ("plain", "plain", "dof")):
+        a.clear()
+        ctx.sync()
+        for _ in range(3):
+            for name in order:
+                (a if name == "dof" else b).render(ctx)
+        ctx.sync()
+        assert same(a.read(), want[order[-1]]), order
+    d.close()
