@@ -85,19 +85,30 @@ struct FDiv {
         const float r0 = __builtin_amdgcn_rcpf(den);
         r = __builtin_fmaf(__builtin_fmaf(-den, r0, 1.0f), r0, r0);
     }
-    SW_D float operator()(float a) const
+    SW_D float operator()(float a) const { return a == 0.0f ? a * r : nonzero(a); }
+    // the quotient of a nonzero numerator in the range above
+    SW_D float nonzero(float a) const
     {
         const float q0 = a * r;
         float q = __builtin_fmaf(__builtin_fmaf(-b, q0, a), r, q0);
-        q = __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
-        return a == 0.0f ? q0 : q;
+        return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
     }
 };
 
-// vector_t::normalize (points.hpp:80-90)
+// vector_t::normalize (points.hpp:80-90).  When every component is at
+// least 2^-60 in magnitude and the length l at most 2^40, the compiler's
+// expansion of each x / l scales nothing (l and 1/l normal, the numerator's
+// exponent within 96 of l's and above 2^-103, the quotient at least 2^-100),
+// so the three quotients share l's refined reciprocal (FDiv's steps, here
+// without its zero case); otherwise the plain divisions.
 SW_D f3 normalize(f3 a)
 {
     float l = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+    const float m = fminf(fabsf(a.x), fminf(fabsf(a.y), fabsf(a.z)));
+    if (m >= 0x1p-60f && l <= 0x1p40f) {
+        const FDiv by(l);
+        return {by.nonzero(a.x), by.nonzero(a.y), by.nonzero(a.z)};
+    }
     if (l != 0) {
         a.x /= l;
         a.y /= l;
