@@ -176,12 +176,14 @@ def fp64_reshade_jobs(ctx, A, check, L, jobs, label):
             "frac_exact": exact / max(px, 1), "depth_and_ids_bit_exact": same}
 
 
-def plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode="graph"):
+def plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode="graph", tunings=None):
     """Render every (scene, viewports) pair resident in HBM two ways:
     plain = one graph replay per scene (all its viewports; plain_mode
     "eager": one swegl_render call per scene instead), batched = one graph
     replay for the whole set.  A frame is one viewport.  Each form carries
-    its HBM roofline: the jobs' algorithmic bytes over the measured time."""
+    its HBM roofline: the jobs' algorithmic bytes over the measured time.
+    tunings: {"plain": entry, "batched": entry} of tuning.json applied
+    before each form, each form timed with the settings tuned for it."""
     from swegl_amd.device import DeviceScene, Target
     from swegl_amd.host import SceneDescHolder
 
@@ -215,7 +217,10 @@ def plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode="graph"):
     frags = st[0].fragments
     out = {"scenes": count, "frames": n_frames, "triangles": int(sum(s.n_triangles() for s, _ in pairs)),
            "pixels": int(sum(vp.m_w * vp.m_h for _, v in pairs for vp in v)), "fragments": int(frags)}
+    tunings = tunings or {}
     for name, fn in (("plain", plain), ("batched", batched)):
+        if tunings.get(name):
+            ctx.apply_tuning(tunings[name])
         fn()
         fn()
         ctx.sync()
@@ -227,6 +232,9 @@ def plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode="graph"):
         out[name] = {"frames_per_s": n_frames * iters / el, "gfragments_per_s": frags * iters / el / 1e9,
                      "ms_per_batch": el * 1e3 / iters}
     out["plain"]["path"] = "graph replay per scene" if plain_mode == "graph" else "swegl_render per scene"
+    for name in ("plain", "batched"):
+        if tunings.get(name):
+            out[name]["tuning"] = {k: tunings[name][k] for k in ("tile", "bin_limit", "raster_groups")}
     out["batched_over_plain"] = out["batched"]["frames_per_s"] / out["plain"]["frames_per_s"]
     # algorithmic bytes of the whole set (shaded pixels read back from the depth planes)
     nbytes = 0
@@ -317,7 +325,7 @@ def bench_demo(ctx, A, check, L, seed, frames):
     return out, scene
 
 
-def bench_batch(ctx, A, check, L, count, seed, iters, plain_mode="graph"):
+def bench_batch(ctx, A, check, L, count, seed, iters, plain_mode="graph", tunings=None):
     """Workload 'batch': `count` height fields (T log-uniform in [1e3, 2e5],
     square framebuffers of 256..1024, DoF on half)."""
     from swegl_amd import scenes
@@ -325,12 +333,12 @@ def bench_batch(ctx, A, check, L, count, seed, iters, plain_mode="graph"):
     t0 = time.perf_counter()
     pairs = scenes.batch_scenes(seed, count=count)
     gen_s = time.perf_counter() - t0
-    out = plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode)
+    out = plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode, tunings)
     out["generate_s"] = round(gen_s, 1)
     return out, pairs
 
 
-def bench_heavy(ctx, A, check, L, count, seed, iters, plain_mode="graph"):
+def bench_heavy(ctx, A, check, L, count, seed, iters, plain_mode="graph", tunings=None):
     """Workload 'heavy': `count` scenes of 500 stacked 1000-triangle
     clusters (500,000 triangles), 4096^2 texture, sun + 8 point lights,
     each seen by 4 viewports of 1920x1080 with the 9-tap DoF."""
@@ -339,19 +347,23 @@ def bench_heavy(ctx, A, check, L, count, seed, iters, plain_mode="graph"):
     t0 = time.perf_counter()
     pairs = scenes.heavy_scenes(seed, count=count)
     gen_s = time.perf_counter() - t0
-    out = plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode)
+    out = plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode, tunings)
     out["generate_s"] = round(gen_s, 1)
     out["workload"] = (f"{count} scenes x 500,000 triangles (500 clusters of 1000, stacks of 8-20), "
                        "4 x 1920x1080 viewports each, tex 4096^2, sun + 8 point lights, 9-tap DoF r_max 8")
     return out
 
 
-def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fracs, plain_mode="graph"):
+def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fracs, plain_mode="graph",
+                 tunings=None):
     """Workload 'served': n_req requests, 95% drawn from the batch scenes and
     5% of the 1M-triangle 1080p shape, each with a seeded camera pose,
     admitted against a 200 GB budget.  Reports the plain path (each request
     rendered alone, one graph replay each), the saturated served rate, and
-    p50/p99 latency at Poisson arrival rates that are fractions of it."""
+    p50/p99 latency at Poisson arrival rates that are fractions of it.
+    tunings: {"plain": entry, "batched": entry} as in plain_vs_batched, the
+    batched entry for the server."""
+    tunings = tunings or {}
     import numpy as np
     from swegl_amd.device import DeviceScene, Server, Target
     from swegl_amd.host import SceneDescHolder
@@ -401,6 +413,12 @@ def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fr
             fv[k] = shaded_pixels(target(allp[k][1][0], i)[1])
     ctx.sync()
     req_bytes = sum(job_bytes(allp[int(k)][0], allp[int(k)][1][:1], [fv[int(k)]]) for k in which)
+    if tunings.get("plain"):
+        ctx.apply_tuning(tunings["plain"])
+        for k in fv:   # warm the graphs of the plain settings
+            i = int(np.argmax(which == k))
+            check(plain_fn(ctx.h, ds[k].h, C.pointer(descs[i]), 1, None), "warm")
+        ctx.sync()
     a = time.perf_counter()
     for i in range(n_req):
         check(plain_fn(ctx.h, ds[int(which[i])].h, C.pointer(descs[i]), 1, None), "plain")
@@ -412,6 +430,8 @@ def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fr
            "hbm_budget_bytes": 200 * 10**9, "algorithmic_bytes": req_bytes,
            "plain_alone": {"frames_per_s": plain_fps, "roofline": roofline_of(req_bytes, plain_s),
                            "path": "graph replay per request" if plain_mode == "graph" else "swegl_render per request"}}
+    if tunings.get("batched"):
+        ctx.apply_tuning(tunings["batched"])
     srv = Server(ctx, 200 * 10**9, max_batch=max_batch, lanes=lanes)
     # saturated: everything queued at t=0
     lat = srv.run_trace(jobs, np.zeros(n_req))
@@ -619,18 +639,19 @@ def main():
         out["cube"], cube = bench_cube(ctx, A, check, L, args.seed, args.cube)
     if args.demo:
         out["demo"], demo_scene = bench_demo(ctx, A, check, L, args.seed, 400)
+    # plain legs (one render per scene or request) with the single-render
+    # tuning, batched and served legs with the batch tuning
+    tunings = {"plain": tuning.get("heightfield"), "batched": tuning.get("batch")}
     if args.batch:
-        if "batch" in tuning:
-            ctx.apply_tuning(tuning["batch"])
         out["batch"], pairs = bench_batch(ctx, A, check, L, args.batch, args.seed + 1, args.batch_iters,
-                                          args.plain)
+                                          args.plain, tunings)
         if args.requests:
             out["served"] = bench_served(ctx, A, check, L, pairs, (scene, vps), args.requests,
-                                         args.seed + 2, args.lanes, args.max_batch, (0.3, 0.6, 0.9), args.plain)
+                                         args.seed + 2, args.lanes, args.max_batch, (0.3, 0.6, 0.9), args.plain,
+                                         tunings)
     if args.heavy:
-        if "heavy" in tuning:
-            ctx.apply_tuning(tuning["heavy"])
-        out["heavy"] = bench_heavy(ctx, A, check, L, args.heavy, args.seed + 3, args.heavy_iters, args.plain)
+        out["heavy"] = bench_heavy(ctx, A, check, L, args.heavy, args.seed + 3, args.heavy_iters, args.plain,
+                                   dict(tunings, batched=tuning.get("heavy", tuning.get("batch"))))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, vps, args.cpu_seconds, args.cpu_threads)
         if cube is not None:

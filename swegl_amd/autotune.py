@@ -25,7 +25,7 @@ from .host import SceneDescHolder
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_PATH = os.path.join(HERE, "tuning.json")
 
-TILES = [(16, 16), (32, 16), (32, 32), (64, 32)]
+TILES = [(16, 16), (32, 16), (32, 32), (64, 32), (64, 64)]
 BIN_LIMITS = [4, 16, 64]
 GROUPS = [2, 4, 8]
 

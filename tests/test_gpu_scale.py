@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 from swegl_amd import _abi as A
-from swegl_amd import scenes
+from swegl_amd import autotune, scenes
 from swegl_amd.device import Context, DeviceScene, Server, Target, check, lib
 from swegl_amd.host import SceneDescHolder
 
@@ -51,13 +51,20 @@ class Alone:
 
 def test_full_mixed_batch_equals_plain(ctx):
     """All 256 scenes of the mixed batch (1,000-200,000 triangles, sides
-    256-1024, DoF on half) in one batched graph equal each scene alone."""
+    256-1024, DoF on half) in one batched graph, with the settings tuned for
+    batches (tile, bin limit, workgroups per CU), equal each scene alone
+    with the settings tuned for single renders."""
+    tuning = autotune.load()
     pairs = scenes.batch_scenes(1)
     assert len(pairs) == 256
     print(f"batch of {len(pairs)} scenes, {sum(s.n_triangles() for s, _ in pairs)} triangles", flush=True)
+    if "batch" in tuning:
+        ctx.apply_tuning(tuning["batch"])
     b = Batch(ctx, pairs)
     b.render(graph=True, stats=False)
     b.render(graph=True, stats=False)   # a replay of the captured graph
+    if "heightfield" in tuning:
+        ctx.apply_tuning(tuning["heightfield"])
     alone = Alone(ctx)
     for k, (s, vps) in enumerate(pairs):
         got = [GpuFrame(c.read(), z.read(), t.read(), None) for c, z, t in b.targets[k]]
