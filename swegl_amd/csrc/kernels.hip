@@ -1900,11 +1900,12 @@ __device__ __forceinline__ uint32_t div_count(uint32_t x, uint32_t c)
     return c == 1 ? x : __umulhi(x, k_dof_recip.m[c]);
 }
 
-// LDS bytes of one DoF workgroup for radius R: one tap word per cell of
-// the output tile plus an R halo on every side.
-__host__ __device__ inline size_t dof_lds_bytes(int R)
+// LDS bytes of a k_dof<RMAX> workgroup: two tap words per cell of the
+// output tile plus an RMAX halo on every side (20 KB for RMAX 8: eight
+// workgroups on a CU).
+__host__ __device__ inline size_t dof_lds_bytes(int RMAX)
 {
-    return (size_t)(SW_DOF_TW + 2 * R) * (SW_DOF_TH + 2 * R) * 4 + 4;
+    return (size_t)(SW_DOF_TW + 2 * RMAX) * (SW_DOF_TH + 2 * RMAX) * 8;
 }
 
 // 256 threads as 64 columns x 4 rows.  Consecutive workgroups run on
@@ -1914,14 +1915,15 @@ __host__ __device__ inline size_t dof_lds_bytes(int R)
 // factor is nonzero (an "unfocused" tap) and its clamped radius.
 // RMAX bounds the wave's halo (the loads below are unrolled for it): the
 // launch picks 8 when no job of the launch has a larger radius.
-// Each LDS cell holds a tap word: an unfocused pixel's b, g, r with 1 in
-// the alpha byte, 0 for a focused pixel or a cell outside the viewport
-// (never an unfocused tap, which is what the reference's clipped loops
-// give), so a tap is one LDS read and its count comes with the colour.
+// Each LDS cell holds a tap as two words, (b, r) and (g, 1) as 16-bit
+// halves for an unfocused pixel, zeros for a focused pixel or a cell
+// outside the viewport (never an unfocused tap, which is what the
+// reference's clipped loops give): a tap is one 64-bit LDS read and two
+// adds, its count coming with the colour.
 template <int RMAX>
 __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restrict__ jobs)
 {
-    extern __shared__ uint32_t s_dyn[];
+    extern __shared__ uint2 s_dyn[];
     const JobDev &J = jobs[find_job<JD>(B, jobs, blockIdx.x)];
     const uint32_t n = dof_blocks(J.V.vw, J.V.vh), q = n / 8;
     uint32_t lb = blockIdx.x - J.dblock0;
@@ -1931,41 +1933,32 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restric
     const int ox = (int)(lb % bx) * SW_DOF_TW, oy = (int)(lb / bx) * SW_DOF_TH;   // viewport-relative
     const int R = J.rmax;
     const int SW = SW_DOF_TW + 2 * R, SH = SW_DOF_TH + 2 * R;
-    uint32_t *s_tap = s_dyn;
+    // The LDS window has the pitch of the largest halo (P) and SH rows; its
+    // cells are dealt to the threads in order, NC / 256 each (the window of
+    // the largest halo is a multiple of 256 cells), so no lane idles on a
+    // ragged row.
+    constexpr int P = SW_DOF_TW + 2 * RMAX, NC = P * (SW_DOF_TH + 2 * RMAX), PERT = NC / 256;
+    static_assert(NC % 256 == 0, "the LDS window is dealt out in whole rounds of 256 cells");
+    uint2 *s_tap = s_dyn;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     // Every load of the halo is issued before the first is used, so the
     // workgroup waits for memory once rather than once per row.  A cell
-    // outside the viewport loads the tile's first pixel instead (no branch
-    // per cell) and is zeroed after.  Offsets are 32-bit: a surface holds
-    // at most 2^30 words.
-    constexpr int ROWS = (SW_DOF_TH + 2 * RMAX + 3) / 4;
-    constexpr int COLS = (SW_DOF_TW + 2 * RMAX + 63) / 64;
+    // outside the viewport (or past the window's SW columns) loads the
+    // tile's first pixel instead (no branch per cell) and is zeroed after.
+    // Offsets are 32-bit: a surface holds at most 2^30 words.
     const SW_RO uint32_t *colr = ro(J.out_color);
     const SW_RO uint8_t *code = ro(J.dofc);
     const int cbase = (J.V.vy - J.out_oy) * J.out_pitch + (J.V.vx - J.out_ox);   // colour of viewport (0, 0)
     const int safe_c = cbase + oy * J.out_pitch + ox, safe_k = oy * J.V.vw + ox;
-    int lxs[COLS];
-    bool cok[COLS];
+    uint32_t cv[PERT], kv[PERT];
+    bool okv[PERT];
 #pragma unroll
-    for (int m = 0; m < COLS; m++) {
-        const int c = tx + 64 * m;
-        lxs[m] = ox - R + c;
-        cok[m] = c < SW && lxs[m] >= 0 && lxs[m] < J.V.vw;
-    }
-    uint32_t cv[ROWS][COLS];
-    uint32_t kv[ROWS][COLS];
-    bool okv[ROWS][COLS];
-#pragma unroll
-    for (int k = 0; k < ROWS; k++) {
-        const int r = ty + 4 * k, ly = oy - R + r;
-        const bool rok = r < SH && ly >= 0 && ly < J.V.vh;
-        const int crow = cbase + ly * J.out_pitch, krow = ly * J.V.vw;
-#pragma unroll
-        for (int m = 0; m < COLS; m++) {
-            okv[k][m] = rok && cok[m];
-            cv[k][m] = colr[okv[k][m] ? crow + lxs[m] : safe_c];   // the raster's output: read-only here
-            kv[k][m] = code[okv[k][m] ? krow + lxs[m] : safe_k];
-        }
+    for (int k = 0; k < PERT; k++) {
+        const int i = (int)threadIdx.x + 256 * k, r = i / P, c = i % P;
+        const int ly = oy - R + r, lx = ox - R + c;
+        okv[k] = r < SH && c < SW && ly >= 0 && ly < J.V.vh && lx >= 0 && lx < J.V.vw;
+        cv[k] = colr[okv[k] ? cbase + ly * J.out_pitch + lx : safe_c];   // the raster's output: read-only here
+        kv[k] = code[okv[k] ? ly * J.V.vw + lx : safe_k];
     }
     // the output pixels' own colours and codes (in the rows just loaded:
     // cache hits)
@@ -1979,15 +1972,9 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restric
         own[j] = code[in ? y * J.V.vw + x : safe_k];
     }
 #pragma unroll
-    for (int k = 0; k < ROWS; k++) {
-        const int r = ty + 4 * k;
-#pragma unroll
-        for (int m = 0; m < COLS; m++) {
-            const int c = tx + 64 * m;
-            if (r < SH && c < SW)
-                s_tap[r * SW + c] =
-                    okv[k][m] && (kv[k][m] & 1u) ? ((cv[k][m] & 0x00FFFFFFu) | (1u << 24)) : 0u;
-        }
+    for (int k = 0; k < PERT; k++) {
+        const uint32_t w = okv[k] && (kv[k] & 1u) ? ((cv[k] & 0x00FFFFFFu) | (1u << 24)) : 0u;
+        s_tap[threadIdx.x + 256 * k] = make_uint2(w & 0x00FF00FFu, (w >> 8) & 0x00FF00FFu);
     }
     __syncthreads();
 #pragma unroll
@@ -1996,7 +1983,7 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restric
         const int x = ox + px, y = oy + py;
         if (x >= J.V.vw || y >= J.V.vh)
             continue;
-        const int c = (py + R) * SW + (px + R);
+        const int c = (py + R) * P + (px + R);
         const int radius = (int)(own[j] >> 1);
         uint32_t out = src[j];
         if (radius != 0) {
@@ -2006,23 +1993,23 @@ __global__ __launch_bounds__(256) void k_dof(BatchDev B, const JobDev *__restric
             constexpr bool PACKED = RMAX <= 8;
             uint32_t br = 0, gc = 0, b = 0, g = 0, r = 0, count = 0;
             auto tap = [&](int k) {   // LDS cell k
-                const uint32_t w = s_tap[k];
+                const uint2 w = s_tap[k];
                 if (PACKED) {
-                    br += w & 0x00FF00FFu;
-                    gc += (w >> 8) & 0x00FF00FFu;
+                    br += w.x;
+                    gc += w.y;
                 } else {
-                    b += w & 0xFF;
-                    g += (w >> 8) & 0xFF;
-                    r += (w >> 16) & 0xFF;
-                    count += w >> 24;
+                    b += w.x & 0xFFFFu;
+                    r += w.x >> 16;
+                    g += w.y & 0xFFFFu;
+                    count += w.y >> 16;
                 }
             };
             if (J.post == 1) {   // box [x-r, x+r) x [y-r, y+r)
                 for (int a = -radius; a < radius; a++)
                     for (int e = -radius; e < radius; e++)
-                        tap(c + a * SW + e);
+                        tap(c + a * P + e);
             } else {             // 3x3 taps at {-r, 0, r}
-                const int dr = radius * SW;
+                const int dr = radius * P;
 #pragma unroll
                 for (int a = -1; a <= 1; a++)
 #pragma unroll
@@ -2138,9 +2125,10 @@ hipError_t launch_dof(const BatchDev &B, uint32_t blocks, int max_radius, hipStr
     if (!blocks)
         return hipSuccess;
     if (max_radius <= 8)
-        LAUNCH_CHECK(k_dof<8>, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B, B.jobs);
+        LAUNCH_CHECK(k_dof<8>, dim3(blocks), dim3(256), dof_lds_bytes(8), s, B, B.jobs);
     else
-        LAUNCH_CHECK(k_dof<SW_MAX_DOF_RADIUS>, dim3(blocks), dim3(256), dof_lds_bytes(max_radius), s, B, B.jobs);
+        LAUNCH_CHECK(k_dof<SW_MAX_DOF_RADIUS>, dim3(blocks), dim3(256), dof_lds_bytes(SW_MAX_DOF_RADIUS), s, B,
+                     B.jobs);
     return hipSuccess;
 }
 
