@@ -204,20 +204,16 @@ __device__ SVert clip_vertex(const JobDev &J, uint32_t a, uint32_t b, float cut)
 
 // fill_triangle (renderer.cpp:240-359): the sub-triangles handed to
 // fill_triangle_2, in call order.  Returns how many (0, 1 or 2).
-// tri_subs_p: triangle (i0, i1, i2) with its vertices' frustum coordinates
-// already loaded (p0, p1, p2), marks not looked at.
-__device__ __forceinline__ int tri_subs_p(const JobDev &J, uint32_t i0, uint32_t i1, uint32_t i2, f3 p0, f3 p1,
-                                          f3 p2, SubTri *out)
+// BINNED: t came out of the binning, which lists a triangle only if all
+// three of its vertices are marked (quick_extent, and this function with
+// marks checked on the exact path), so the marks need no second look.
+// The same with triangle t's vertex ids already loaded (i0, i1, i2).
+template <bool BINNED = false>
+__device__ int tri_subs_v(const JobDev &J, uint32_t i0, uint32_t i1, uint32_t i2, SubTri *out)
 {
-    SVert v0, v1, v2;
-    v0.s = frustum_to_screen(J.V, p0);
-    v1.s = frustum_to_screen(J.V, p1);
-    v2.s = frustum_to_screen(J.V, p2);
-    v0.a = v0.b = i0;
-    v1.a = v1.b = i1;
-    v2.a = v2.b = i2;
-    v0.cut = v1.cut = v2.cut = 0.0f;
-    v0.clip = v1.clip = v2.clip = 0;
+    if (!BINNED && !(ro(J.yes)[i0] && ro(J.yes)[i1] && ro(J.yes)[i2]))
+        return 0;
+    SVert v0 = orig_vertex(J, i0), v1 = orig_vertex(J, i1), v2 = orig_vertex(J, i2);
     bool front = cross_z_sign(v1.s - v0.s, v2.s - v0.s, -1);
     bool inv = false;
     if (v1.s.z > v0.s.z) { SVert x = v0; v0 = v1; v1 = x; inv = !inv; }
@@ -260,17 +256,6 @@ __device__ __forceinline__ int tri_subs_p(const JobDev &J, uint32_t i0, uint32_t
         return 2;
     }
     return 0;   // NaN depth: no branch of the reference is taken
-}
-// The same with triangle t's vertex ids already loaded (i0, i1, i2).
-// BINNED: t came out of the binning, which lists a triangle only if all
-// three of its vertices are marked (quick_extent, and this function with
-// marks checked on the exact path), so the marks need no second look.
-template <bool BINNED = false>
-__device__ int tri_subs_v(const JobDev &J, uint32_t i0, uint32_t i1, uint32_t i2, SubTri *out)
-{
-    if (!BINNED && !(ro(J.yes)[i0] && ro(J.yes)[i1] && ro(J.yes)[i2]))
-        return 0;
-    return tri_subs_p(J, i0, i1, i2, ld3(J.ndc, i0), ld3(J.ndc, i1), ld3(J.ndc, i2), out);
 }
 template <bool BINNED = false>
 __device__ __forceinline__ int tri_subs(const JobDev &J, uint32_t t, SubTri *out)
@@ -880,13 +865,17 @@ __device__ __forceinline__ void walk_rows(const Setup &S, int wy0, int wy1, RowF
     }
 }
 
-// raster_subs: the walk of triangle t's sub-triangles st[0, ns) (vertex ids
-// i0, i1, i2), from wherever they were loaded; raster_depth below loads them.
-__device__ __forceinline__ void raster_subs(const JobDev &J, uint32_t t, uint32_t i0, uint32_t i1, uint32_t i2,
-                                            const SubTri *st, int ns, TileRect tr, int wy0, int wy1,
-                                            unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans,
-                                            unsigned long long *slots, int nslot)
+// check_mark: t came from a list that holds unmarked triangles too (the
+// marks are read beside the vertices, the triangle skipped when unmarked).
+__device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, int wy1,
+                             unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans,
+                             unsigned long long *slots = nullptr, int nslot = 0, bool check_mark = false)
 {
+    SubTri st[2];
+    const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
+    if (check_mark && !((ro(J.yes)[i0] != 0) & (ro(J.yes)[i1] != 0) & (ro(J.yes)[i2] != 0)))
+        return;
+    int ns = tri_subs_v<true>(J, i0, i1, i2, st);
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         Setup S;
@@ -927,20 +916,6 @@ __device__ __forceinline__ void raster_subs(const JobDev &J, uint32_t t, uint32_
             }
         });
     }
-}
-
-// check_mark: t came from a list that holds unmarked triangles too (the
-// marks are read beside the vertices, the triangle skipped when unmarked).
-__device__ void raster_depth(const JobDev &J, uint32_t t, TileRect tr, int wy0, int wy1,
-                             unsigned long long *keys, int TW, uint32_t &frags, uint32_t &nans,
-                             unsigned long long *slots = nullptr, int nslot = 0, bool check_mark = false)
-{
-    SubTri st[2];
-    const uint32_t i0 = ro(J.tri)[3 * t], i1 = ro(J.tri)[3 * t + 1], i2 = ro(J.tri)[3 * t + 2];
-    if (check_mark && !((ro(J.yes)[i0] != 0) & (ro(J.yes)[i1] != 0) & (ro(J.yes)[i2] != 0)))
-        return;
-    const int ns = tri_subs_v<true>(J, i0, i1, i2, st);
-    raster_subs(J, t, i0, i1, i2, st, ns, tr, wy0, wy1, keys, TW, frags, nans, slots, nslot);
 }
 
 // Shading attributes of a sub-triangle vertex (v_world, normal_world, raw tex
@@ -1179,26 +1154,22 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
     const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
     if (nb + nbig > 256) {
-        // enough triangles to fill the lanes: one lane per triangle, all rows.
-        // Each bin entry is loaded one trip ahead, so that the first of an
-        // entry's three dependent loads runs while the lane walks the entry
-        // before (ahead of the vertex ids too, the walk's registers spilled).
-        uint32_t e = threadIdx.x;
-        uint32_t nt = e < nb ? ro(B.bins)[off + e] : 0u;
-        for (; e < nb; e += 256) {
-            const uint32_t t = nt & ~SW_BIN_CHECK;
-            const bool check = (nt & SW_BIN_CHECK) != 0;   // never set in the filled layout
-            if (e + 256 < nb)
-                nt = ro(B.bins)[off + e + 256];
+        // enough triangles to fill the lanes: one lane per triangle, all rows
+        for (uint32_t e = threadIdx.x; e < nb + nbig; e += 256) {
+            uint32_t t;
+            bool check = false;
+            if (e < nb) {
+                t = ro(B.bins)[off + e];
+                check = (t & SW_BIN_CHECK) != 0;   // never set in the filled layout
+                t &= ~SW_BIN_CHECK;
+            } else {
+                t = ro(J.big)[e - nb];
+                const uint2 r = ro2(J.rect, t);
+                if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
+                    ty > (int)(r.y >> 16) || !marked(J, t))
+                    continue;
+            }
             raster_depth(J, t, tr, tr.y0, tr.y1, keys, TW, frags, nans, slots, NSLOT, check);
-        }
-        for (; e < nb + nbig; e += 256) {   // the job's big triangles that touch this tile
-            const uint32_t t = ro(J.big)[e - nb];
-            const uint2 r = ro2(J.rect, t);
-            if (tx < (int)(r.x & 0xFFFF) || tx > (int)(r.y & 0xFFFF) || ty < (int)(r.x >> 16) ||
-                ty > (int)(r.y >> 16) || !marked(J, t))
-                continue;
-            raster_depth(J, t, tr, tr.y0, tr.y1, keys, TW, frags, nans, slots, NSLOT);
         }
     } else {
     for (uint32_t base = 0; base < nb + nbig; base += 256) {
