@@ -251,7 +251,8 @@ def plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode="graph", tunings
 
 def bench_cube(ctx, A, check, L, seed, frames):
     """Workload 'cube': make_cube(1, 0) at seeded Euler angles, 256^2 texture,
-    one sun, 640x480, plain path (one render call per frame, eager)."""
+    one sun, 640x480, plain path (one render call per frame, eager), and the
+    same frames as one graph replay each."""
     import numpy as np
     from swegl_amd import scenes
     from swegl_amd.device import DeviceScene, Target
@@ -274,6 +275,15 @@ def bench_cube(ctx, A, check, L, seed, frames):
         check(L.swegl_render(ctx.h, ds.h, d, 1, None), "swegl_render")
     ctx.sync()
     el = time.perf_counter() - a
+    # the same frames as one graph replay each (the captured launch sequence)
+    for _ in range(3):
+        check(L.swegl_render_graph(ctx.h, ds.h, d, 1, None), "swegl_render_graph")
+    ctx.sync()
+    a = time.perf_counter()
+    for _ in range(frames):
+        check(L.swegl_render_graph(ctx.h, ds.h, d, 1, None), "swegl_render_graph")
+    ctx.sync()
+    el_graph = time.perf_counter() - a
     lat = []
     for _ in range(100):
         t = time.perf_counter()
@@ -282,7 +292,8 @@ def bench_cube(ctx, A, check, L, seed, frames):
         lat.append((time.perf_counter() - t) * 1e3)
     ds.close()
     return {"workload": f"cube 12 triangles, 640x480, tex 256^2, one sun, seed {seed}",
-            "frames_per_s": frames / el, "gfragments_per_s": st[0].fragments * frames / el / 1e9,
+            "frames_per_s": frames / el, "graph_frames_per_s": frames / el_graph,
+            "gfragments_per_s": st[0].fragments * frames / el / 1e9,
             "fragments_per_frame": int(st[0].fragments),
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}}, (scene, vps)
 
