@@ -204,10 +204,15 @@ def plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode="graph", tunings
     n_frames = sum(len(d) for d in descs)
 
     plain_fn = L.swegl_render_graph if plain_mode == "graph" else L.swegl_render
+    other_fn = L.swegl_render if plain_mode == "graph" else L.swegl_render_graph
 
     def plain():
         for k in range(count):
             check(plain_fn(ctx.h, ds[k].h, descs[k], len(descs[k]), None), "render_plain")
+
+    def plain_other():
+        for k in range(count):
+            check(other_fn(ctx.h, ds[k].h, descs[k], len(descs[k]), None), "render_plain")
 
     def batched():
         check(L.swegl_render_batch_graph(ctx.h, jobs, count, None), "render_batch_graph")
@@ -218,9 +223,9 @@ def plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode="graph", tunings
     out = {"scenes": count, "frames": n_frames, "triangles": int(sum(s.n_triangles() for s, _ in pairs)),
            "pixels": int(sum(vp.m_w * vp.m_h for _, v in pairs for vp in v)), "fragments": int(frags)}
     tunings = tunings or {}
-    for name, fn in (("plain", plain), ("batched", batched)):
-        if tunings.get(name):
-            ctx.apply_tuning(tunings[name])
+    for name, fn in (("plain", plain), ("plain_other", plain_other), ("batched", batched)):
+        if tunings.get(name if name != "plain_other" else "plain"):
+            ctx.apply_tuning(tunings[name if name != "plain_other" else "plain"])
         fn()
         fn()
         ctx.sync()
@@ -232,6 +237,13 @@ def plain_vs_batched(ctx, A, check, L, pairs, iters, plain_mode="graph", tunings
         out[name] = {"frames_per_s": n_frames * iters / el, "gfragments_per_s": frags * iters / el / 1e9,
                      "ms_per_batch": el * 1e3 / iters}
     out["plain"]["path"] = "graph replay per scene" if plain_mode == "graph" else "swegl_render per scene"
+    # the other single-render path beside it (eager renders pipeline one
+    # scene's front end under the previous scene's raster)
+    other = out.pop("plain_other")
+    other["path"] = "swegl_render per scene" if plain_mode == "graph" else "graph replay per scene"
+    out["plain_" + ("eager" if plain_mode == "graph" else "graph")] = other
+    out["batched_over_plain_" + ("eager" if plain_mode == "graph" else "graph")] = (
+        out["batched"]["frames_per_s"] / other["frames_per_s"])
     for name in ("plain", "batched"):
         if tunings.get(name):
             out[name]["tuning"] = {k: tunings[name][k] for k in ("tile", "bin_limit", "raster_groups")}
@@ -435,12 +447,26 @@ def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fr
         check(plain_fn(ctx.h, ds[int(which[i])].h, C.pointer(descs[i]), 1, None), "plain")
     ctx.sync()
     plain_fps = n_req / (time.perf_counter() - a)
+    # the other single-render path beside it
+    other_fn = L.swegl_render if plain_mode == "graph" else L.swegl_render_graph
+    for k in fv:
+        check(other_fn(ctx.h, ds[k].h, C.pointer(descs[int(np.argmax(which == k))]), 1, None), "warm")
+    ctx.sync()
+    a = time.perf_counter()
+    for i in range(n_req):
+        check(other_fn(ctx.h, ds[int(which[i])].h, C.pointer(descs[i]), 1, None), "plain")
+    ctx.sync()
+    other_fps = n_req / (time.perf_counter() - a)
 
     plain_s = n_req / plain_fps
+    other_name = "eager" if plain_mode == "graph" else "graph"
     out = {"requests": n_req, "large_requests": n_big, "lanes": lanes, "max_batch": max_batch,
            "hbm_budget_bytes": 200 * 10**9, "algorithmic_bytes": req_bytes,
            "plain_alone": {"frames_per_s": plain_fps, "roofline": roofline_of(req_bytes, plain_s),
-                           "path": "graph replay per request" if plain_mode == "graph" else "swegl_render per request"}}
+                           "path": "graph replay per request" if plain_mode == "graph" else "swegl_render per request"},
+           "plain_alone_" + other_name: {
+               "frames_per_s": other_fps,
+               "path": "swegl_render per request" if plain_mode == "graph" else "graph replay per request"}}
     if tunings.get("batched"):
         ctx.apply_tuning(tunings["batched"])
     srv = Server(ctx, 200 * 10**9, max_batch=max_batch, lanes=lanes)
@@ -452,6 +478,7 @@ def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fr
                         "p99_ms": float(np.percentile(lat, 99) * 1e3), "batches": st["batches"],
                         "roofline": roofline_of(req_bytes, float(lat.max()))}
     out["served_over_plain"] = cap_fps / plain_fps
+    out["served_over_plain_" + other_name] = cap_fps / other_fps
     rates = []
     for f in fracs:
         rate = f * cap_fps
