@@ -1053,7 +1053,8 @@ struct Wave {
     bool generic = false;     // some job uses a shader other than Phong + bilinear
     bool layers = false;      // the wave's (single) job has transparency layers
     uint64_t cap = 0;
-    uint32_t tile_cap = 0;    // entries per tile of k_front's fixed layout (BatchDev::tile_cap), 0: none
+    bool fixed = false;       // k_front's fixed layout (BatchDev::fixed; per job JobDev::tile_cap)
+    uint64_t fixed_total = 0; // entries of the fixed layout, ahead of the counted layout in the bins
     uint32_t big_tiles = 0;   // bin limit of this wave (BatchDev::big_tiles)
     size_t job_offset = 0;   // byte offset of this wave's table in Resources::jobs
 };
@@ -1245,23 +1246,40 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         }
         tris_total += s->n_tris;
     }
-    W.n_ctr = SW_CTR_BIG0 + nj;
+    W.n_ctr = SW_CTR_BIG0 + 2 * nj;   // big-list lengths, then fixed-layout overflow words
     W.big_tiles = std::max<uint32_t>(1, W.layers ? std::max<uint32_t>(c->big_tiles, SW_LAYER_BIG_TILES)
                                                  : c->big_tiles);
     // every binned triangle contributes at most big_tiles entries
     W.cap = std::max<uint64_t>(4096, tris_total * W.big_tiles);
     if (W.cap > 0xFFFFFFF0ull)
         return fail(SWEGL_ERANGE, "too many triangles in one batch");
-    // k_front's fixed layout (the context lane's buffers, no layers): room
-    // for eight times a tile's share of 1.25 entries per triangle, at least
-    // 512; a frame whose busiest tile needs more takes the filled layout
-    W.tile_cap = 0;
+    // k_front's fixed layout (the context lane's buffers, no layers): each
+    // job's tiles get room for eight times the job's per-tile share of 1.25
+    // entries per triangle, at least 512 (a batch mixes scenes of very
+    // different densities); a job whose busiest tile needs more takes the
+    // filled layout, the others keep theirs
+    W.fixed = false;
+    W.fixed_total = 0;
     if (!arena && !W.layers && W.tiles && tris_total < 0x80000000ull) {   // (SW_BIN_CHECK is the top bit)
-        const uint64_t per = (tris_total * 10 + W.tiles - 1) / W.tiles;   // 8 x 1.25 entries per triangle
-        const uint64_t tc = (std::max<uint64_t>(512, per) + 63) & ~(uint64_t)63;
-        if ((uint64_t)W.tiles * tc <= 0xFFFFFFF0ull)
-            W.tile_cap = (uint32_t)tc;
+        uint64_t total = 0;
+        for (JobDev &J : W.jd) {
+            const uint64_t tiles = (uint64_t)J.tiles_x * J.tiles_y;
+            const uint64_t per = tiles ? ((uint64_t)J.n_tris * 10 + tiles - 1) / tiles : 0;   // 8 x 1.25
+            const uint64_t tc = (std::max<uint64_t>(512, per) + 63) & ~(uint64_t)63;
+            J.tile_cap = (uint32_t)std::min<uint64_t>(tc, 0xFFFFFFC0ull);
+            J.bin_base = total;
+            total += tiles * J.tile_cap;
+        }
+        if (total <= 0xFFFFFFF0ull) {   // entry offsets are 32-bit
+            W.fixed = true;
+            W.fixed_total = total;
+        }
     }
+    if (!W.fixed)
+        for (JobDev &J : W.jd) {
+            J.tile_cap = 0;
+            J.bin_base = 0;
+        }
     return SWEGL_OK;
 }
 
@@ -1290,7 +1308,7 @@ static void resource_sizes(std::vector<Wave> &waves, size_t sz[5])
     for (Wave &W : waves) {
         tiles = std::max(tiles, W.tiles);
         n_ctr = std::max(n_ctr, W.n_ctr);
-        cap = std::max(cap, std::max<uint64_t>(W.cap, (uint64_t)W.tiles * W.tile_cap));
+        cap = std::max(cap, W.fixed_total + W.cap);   // the fixed layout, then the counted one
     }
     sz[0] = job_bytes;
     sz[1] = (size_t)tiles * SW_TILE_PAD * 4 + 8;
@@ -1346,9 +1364,10 @@ static int launch_wave(swegl_ctx *c, Lane &L, const ResPtr &R, const Wave &W, bo
     B.tile_offset = R.tile_offset;
     B.tile_order = R.tile_offset + W.tiles + 1;
     B.tile_cursor = R.tile_count + 1;
-    B.bins = R.bins;
+    B.bins_fixed = R.bins;
+    B.bins = R.bins + W.fixed_total;
     B.bin_capacity = (uint32_t)W.cap;
-    B.tile_cap = W.tile_cap;
+    B.fixed = W.fixed ? 1u : 0u;
     B.ctr = R.ctr;
     B.big_tiles = W.big_tiles;
     B.stats = want_stats ? 1u : 0u;
@@ -1610,7 +1629,7 @@ static std::string graph_key(swegl_ctx *c, const std::vector<Wave> &waves)
         k += "|" + std::to_string(W.tw) + "x" + std::to_string(W.th) + ":" + std::to_string(W.jd.size()) + ":" +
              std::to_string(W.vblk) + "," + std::to_string(W.tblk) +
              "," + std::to_string(W.dblk) + "," + std::to_string(W.tiles) + "," + std::to_string(W.cap) + "," +
-             std::to_string(W.rmax) + "," + std::to_string(W.big_tiles) + "," + std::to_string(W.tile_cap) +
+             std::to_string(W.rmax) + "," + std::to_string(W.big_tiles) + "," + std::to_string(W.fixed_total) +
              (W.generic ? "g" : "") +
              (W.layers ? "L" : "");
         // the layer waves' flatten and copy grids are sized by the viewport's pixels
@@ -1886,7 +1905,7 @@ static ReqBytes request_bytes(const swegl_ctx *c, const swegl_scene *s, const st
         const uint32_t bt = std::max<uint32_t>(1, layers ? std::max<uint32_t>(c->big_tiles, SW_LAYER_BIG_TILES)
                                                           : c->big_tiles);
         b.scratch += tiles * (SW_TILE_PAD * 4 + 8) + 16 + (uint64_t)std::max<uint32_t>(1, s->n_tris) * bt * 4 +
-                     sizeof(JobDev) + 4;
+                     sizeof(JobDev) + 8;   // the job's table entry and its two counter words
         const uint64_t P = (uint64_t)v.w * v.h;
         b.targets += (uint64_t)v.color->w * v.color->h * 4 + P * 4 + (v.triangle_id ? P * 4 : 0);
     }

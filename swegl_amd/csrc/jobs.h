@@ -97,6 +97,10 @@ struct JobDev {
     // ---- tiles
     int32_t tw, th, tiles_x, tiles_y;
     uint32_t tile_base;       // first global tile id of this job
+    // k_front's fixed layout (BatchDev::fixed): this job's tile k keeps its
+    // first tile_cap entries at bins_fixed[bin_base + k * tile_cap + rank]
+    uint32_t tile_cap;
+    uint64_t bin_base;
     // ---- first block of this job in each batched launch
     uint32_t vblock0, tblock0, dblock0;
 };
@@ -123,22 +127,25 @@ struct BatchDev {
     uint32_t *tile_offset;    // [n_tiles + 1]
     uint32_t *tile_order;     // [n_tiles]: tiles by descending bin count (k_bin_fill's last block), the raster's claim order
     uint32_t *tile_cursor;    // tile_count + 1: the fill ranks
-    uint32_t *bins;           // [capacity]
+    uint32_t *bins;           // [capacity]: the counted, scanned and filled layout
+    uint32_t *bins_fixed;     // the fixed layout's entries (JobDev::bin_base, tile_cap)
     uint32_t bin_capacity;
     uint32_t big_tiles;       // per-triangle bin limit; larger ones go to the big list
     uint32_t *ctr;            // counter words, layout below
     uint32_t stats;           // 1: accumulate the statistics counters
-    // > 0: k_front places every entry at bins[g * tile_cap + rank] as it
-    // counts it (marks unchecked), and k_bin_fill and the offsets scan run
-    // only when a tile holds more entries than that (SW_CTR_OVERFLOW set);
-    // 0: the counted, scanned and filled layout alone
-    uint32_t tile_cap;
+    // 1: k_front places every entry in its job's fixed layout as it counts
+    // it (marks unchecked); only a job with a tile holding more entries than
+    // its tile_cap (its word at SW_CTR_BIG0 + n_jobs + job set, and
+    // SW_CTR_OVERFLOW) is filled by k_bin_fill into the counted layout,
+    // after the offsets scan; 0: the counted layout for every job
+    uint32_t fixed;
 };
 
 // Counter block: [0, 1024) statistics as 64 slots x 8 u64 (slot = block
 // index mod 64, so no address takes one atomic per block; the host sums the
 // slots), then the raster's tile queues (one per 256-byte line), the total
-// bin entry count, and the big-list length of each job j at SW_CTR_BIG0 + j.
+// bin entry count, the big-list length of each job j at SW_CTR_BIG0 + j,
+// and its fixed-layout overflow word at SW_CTR_BIG0 + n_jobs + j.
 #define SW_STAT_SLOTS 64
 #define SW_CTR_STATS 0
 #define SW_CTR_CONTROL 1024

@@ -604,17 +604,20 @@ __global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restr
             const int ky = k == 0 ? 0 : k / w, kx = k == 0 ? 0 : k % w;   // k is the same in every lane
             key = J.tile_base + (ty0 + ky) * J.tiles_x + (tx0 + kx);
         }
-        if (B.tile_cap) {
+        if (B.fixed) {
             const uint32_t rank = wave_add<true>(B.tile_count, key * SW_TILE_PAD, act);
-            if (act && rank < B.tile_cap)   // top bit: the raster must look at the marks
-                B.bins[(size_t)key * B.tile_cap + rank] = self_marked ? t : t | SW_BIN_CHECK;
-            overflow |= act && rank >= B.tile_cap;
+            if (act && rank < J.tile_cap)   // top bit: the raster must look at the marks
+                B.bins_fixed[J.bin_base + (size_t)(key - J.tile_base) * J.tile_cap + rank] =
+                    self_marked ? t : t | SW_BIN_CHECK;
+            overflow |= act && rank >= J.tile_cap;
         } else {
             wave_add<false>(B.tile_count, key * SW_TILE_PAD, act);
         }
     }
-    if (__any(overflow) && __lane_id() == 0)
+    if (__any(overflow) && __lane_id() == 0) {   // every lane of a block has the block's job
+        atomicOr(B.ctr + SW_CTR_BIG0 + B.n_jobs + jidx, 1u);
         atomicOr(B.ctr + SW_CTR_OVERFLOW, 1u);
+    }
     // The last block to finish scans.  The counts are device-scope atomics,
     // performed at the memory side, not in an XCD's L2: each wave waits
     // for its own to complete before the block takes its ticket, and the
@@ -640,7 +643,7 @@ __global__ __launch_bounds__(256) void k_front(BatchDev B, const JobDev *__restr
     if (!s_last)
         return;
     // the offsets serve the filled layout only
-    if (!B.tile_cap || __hip_atomic_load(B.ctr + SW_CTR_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    if (!B.fixed || __hip_atomic_load(B.ctr + SW_CTR_OVERFLOW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         scan_offsets<256>(B);
 }
 
@@ -654,13 +657,14 @@ __global__ __launch_bounds__(256) void k_bin_fill(BatchDev B, const JobDev *__re
         order_tiles<256>(B);
         return;
     }
-    // k_front placed every entry unless a tile overflowed: then only the
-    // statistics, if asked for, are counted here
-    const bool place = !B.tile_cap || B.ctr[SW_CTR_OVERFLOW] != 0;
+    // k_front placed every entry of a job unless one of its tiles
+    // overflowed: then only the statistics, if asked for, are counted here
+    const uint32_t blk = blockIdx.x - 1;
+    const int jidx = find_job<JT>(B, jobs, blk);
+    const bool place = !B.fixed || B.ctr[SW_CTR_BIG0 + B.n_jobs + jidx] != 0;
     if (!place && !B.stats)
         return;
-    const uint32_t blk = blockIdx.x - 1;
-    const JobDev &J = jobs[find_job<JT>(B, jobs, blk)];
+    const JobDev &J = jobs[jidx];
     const uint32_t t = (blk - J.tblock0) * 256u + threadIdx.x;
     int ntiles = 0, tx0 = 0, tx1 = 0, ty0 = 0;
     uint32_t drawn = 0, big = 0;
@@ -1112,8 +1116,7 @@ __device__ uint32_t shade_recorded(const JobDev &J, uint32_t seq, uint32_t tag, 
 
 template <int TW, int TH, bool GENERIC>
 __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__restrict__ jobs, uint32_t g,
-                                            unsigned long long *keys, uint32_t *s_cnt, unsigned long long *slots,
-                                            bool fixed)
+                                            unsigned long long *keys, uint32_t *s_cnt, unsigned long long *slots)
 {
     constexpr int NSLOT = TW * TH;
     const int jidx = find_job<JTILE>(B, jobs, g);
@@ -1138,18 +1141,20 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
     // are dealt to the lanes, so a large triangle does not leave one lane
     // looping over its pixels while the others idle.
     __shared__ uint32_t s_tri[256], s_rows[256], s_first[256], s_wsum[4];
-    // The tile's entries: placed by k_front at g * tile_cap (every triangle
-    // whose rectangle holds the tile; marks are checked here), or, when a
-    // tile passed that capacity, by k_bin_fill at the scanned offset (marked
-    // triangles only; the count also held unmarked ones); then the job's big
-    // list.
-    uint32_t nb, off;
+    // The tile's entries: placed by k_front in its job's fixed layout
+    // (every triangle whose rectangle holds the tile; marks are checked
+    // here), or, when a tile of the job passed its capacity, by k_bin_fill
+    // at the scanned offset (marked triangles only; the count also held
+    // unmarked ones); then the job's big list.
+    const bool fixed = B.fixed != 0 && B.ctr[SW_CTR_BIG0 + B.n_jobs + jidx] == 0;
+    uint32_t nb;
+    const uint32_t *bins;
     if (fixed) {
-        nb = min(B.tile_count[g * SW_TILE_PAD], B.tile_cap);
-        off = g * B.tile_cap;
+        nb = min(B.tile_count[g * SW_TILE_PAD], J.tile_cap);
+        bins = B.bins_fixed + J.bin_base + (size_t)lt * J.tile_cap;
     } else {
         nb = B.tile_cursor[g * SW_TILE_PAD];
-        off = B.tile_offset[g];
+        bins = B.bins + B.tile_offset[g];
     }
     const uint32_t nbig = B.ctr[SW_CTR_BIG0 + jidx];
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
@@ -1159,7 +1164,7 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
             uint32_t t;
             bool check = false;
             if (e < nb) {
-                t = ro(B.bins)[off + e];
+                t = ro(bins)[e];
                 check = (t & SW_BIN_CHECK) != 0;   // never set in the filled layout
                 t &= ~SW_BIN_CHECK;
             } else {
@@ -1179,7 +1184,7 @@ __device__ __forceinline__ void raster_tile(const BatchDev &B, const JobDev *__r
         bool check = false;
         if (e < nb + nbig) {
             if (e < nb) {
-                t = ro(B.bins)[off + e];
+                t = ro(bins)[e];
                 check = (t & SW_BIN_CHECK) != 0;
                 t &= ~SW_BIN_CHECK;
             } else {
@@ -1333,8 +1338,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // index mod 8), then the others in turn; a queue seen exhausted is never
     // asked again, so every workgroup stops after at most n_tiles + 8 claims
     // in total.
-    // which layout the bins have this frame (the same for every tile)
-    const bool fixed = B.tile_cap != 0 && B.ctr[SW_CTR_OVERFLOW] == 0;
     const uint32_t T = B.n_tiles, home = blockIdx.x % SW_RASTER_QUEUES;
     uint32_t tried = 0;   // thread 0 only: queues found exhausted
     for (;;) {
@@ -1360,7 +1363,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const uint32_t g = s_tile;
         if (g == 0xFFFFFFFFu)
             break;
-        raster_tile<TW, TH, GENERIC>(B, jobs, g, keys, s_cnt, SLOTS ? slots : nullptr, fixed);
+        raster_tile<TW, TH, GENERIC>(B, jobs, g, keys, s_cnt, SLOTS ? slots : nullptr);
         __syncthreads();   // tile done with keys and s_tile before the next claim
     }
     __syncthreads();

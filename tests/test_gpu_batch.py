@@ -118,3 +118,28 @@ def test_multi_wave_graph_replays(ctx):
             assert np.array_equal(planes[i][1].read(), eager[i].tid), (rep, i)
             assert np.array_equal(planes[i][0].read().view(np.uint32), eager[i].zbuffer.view(np.uint32)), (rep, i)
     ds.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_one_job_overflows_its_tiles(ctx, graph):
+    """A batch of a crowded height field (19,602 triangles seen from 30 units
+    back: its busiest tiles hold more entries than its share of k_front's
+    fixed layout, so that job alone takes the counted, scanned and filled
+    layout) between mixed-set scenes that keep their fixed layouts: every
+    job equals its render alone, and the crowded one and a neighbour match
+    the oracle."""
+    crowd, vps = scenes.heightfield_scene(2, n=100, width=1920, height=1080, dof=False)
+    vps[0].m_camera.translate(0.0, 0.0, -30.0)
+    pairs = small_batch(4, count=4)
+    pairs = pairs[:2] + [(crowd, vps)] + pairs[2:]
+    b = Batch(ctx, pairs)
+    st = b.render(graph=graph)
+    got = b.frames()
+    for k, (s, v) in enumerate(pairs):
+        alone = gpu_render(ctx, s, v)[0]
+        assert same_frame(got[k][0], alone), k
+    for k in (1, 2):
+        s, v = pairs[k]
+        compare(got[k][0], oracle.render(s, v)[0], colour_tol=1)
+    if not graph:
+        assert st["fragments"] > 0
