@@ -395,12 +395,18 @@ def bench_served(ctx, A, check, L, pairs, big, n_req, seed, lanes, max_batch, fr
     allp = list(pairs) + [big]
     ds = [DeviceScene(ctx, SceneDescHolder(s)) for s, _ in allp]
     pools, keep = {}, []
-    slots = lanes * max_batch + 8
+    # Request i writes target set i mod the pool size of its viewport size.
+    # The trace runner submits in arrival order and holds back a request
+    # whose targets an earlier one still writes, so the pools bound how
+    # many requests wait: four full batches per lane of small ones, 64
+    # large ones (5 % of the stream), so that a saturated run keeps the
+    # lanes' batches full.
+    slots = 4 * lanes * max_batch + 8
 
     def target(vp, i):
         key = (vp.m_w, vp.m_h)
         pool = pools.setdefault(key, [])
-        j = i % (slots if key != (big[1][0].m_w, big[1][0].m_h) else 16)
+        j = i % (slots if key != (big[1][0].m_w, big[1][0].m_h) else 64)
         while len(pool) <= j:
             pool.append((Target(ctx, A.TARGET_COLOR, vp.m_w, vp.m_h),
                          Target(ctx, A.TARGET_FLOAT, vp.m_w, vp.m_h)))
