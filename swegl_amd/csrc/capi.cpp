@@ -1255,17 +1255,19 @@ static int build_wave(swegl_ctx *c, Lane &L, const std::vector<JobIn> &jobs, Wav
         return fail(SWEGL_ERANGE, "too many triangles in one batch");
     // k_front's fixed layout (the context lane's buffers and server lanes'
     // regions, whose request bound counts it; not the allocation-free path's
-    // fixed array; no layers): each job's tiles get room for eight times the
-    // job's per-tile share of 1.25 entries per triangle, at least 512 (a
-    // batch mixes scenes of very different densities); a job whose busiest
-    // tile needs more takes the filled layout, the others keep theirs
+    // fixed array; no layers): each job's tiles get room for sixteen times
+    // the job's per-tile share of 1.25 entries per triangle, at least 512 (a
+    // batch mixes scenes of very different densities, and a crowded tile --
+    // a horizon, a stack of meshes -- holds 8-19 times its job's share); a
+    // job whose busiest tile needs more takes the filled layout, the others
+    // keep theirs
     W.fixed = false;
     W.fixed_total = 0;
     if ((!arena || L.own_views) && !W.layers && W.tiles && tris_total < 0x80000000ull) {   // (SW_BIN_CHECK: top bit)
         uint64_t total = 0;
         for (JobDev &J : W.jd) {
             const uint64_t tiles = (uint64_t)J.tiles_x * J.tiles_y;
-            const uint64_t per = tiles ? ((uint64_t)J.n_tris * 10 + tiles - 1) / tiles : 0;   // 8 x 1.25
+            const uint64_t per = tiles ? ((uint64_t)J.n_tris * 20 + tiles - 1) / tiles : 0;   // 16 x 1.25
             const uint64_t tc = (std::max<uint64_t>(512, per) + 63) & ~(uint64_t)63;
             J.tile_cap = (uint32_t)std::min<uint64_t>(tc, 0xFFFFFFC0ull);
             J.bin_base = total;
@@ -1908,10 +1910,10 @@ static ReqBytes request_bytes(const swegl_ctx *c, const swegl_scene *s, const st
         b.scratch += tiles * (SW_TILE_PAD * 4 + 8) + 16 + (uint64_t)std::max<uint32_t>(1, s->n_tris) * bt * 4 +
                      sizeof(JobDev) + 8;   // the job's table entry and its two counter words
         // its share of the fixed bin layout (build_wave): tiles x cap entries
-        // with cap <= max(512, 10 n_tris / tiles + 1) + 63, largest at the
+        // with cap <= max(512, 20 n_tris / tiles + 1) + 63, largest at the
         // smallest tile
         if (!layers)
-            b.scratch += (tiles * 640 + (uint64_t)s->n_tris * 10) * 4;
+            b.scratch += (tiles * 640 + (uint64_t)s->n_tris * 20) * 4;
         const uint64_t P = (uint64_t)v.w * v.h;
         b.targets += (uint64_t)v.color->w * v.color->h * 4 + P * 4 + (v.triangle_id ? P * 4 : 0);
     }
